@@ -1,0 +1,198 @@
+/*
+ * orctm_nh.h -- C ABI of the MI355X (gfx950) nonhydrostatic pressure Poisson solver.
+ *
+ * Drop-in for ORCTM's PETSc path (ORCTM/src_orctm/mo_petsc.f90, module MO_PETSC):
+ *
+ *   reference call                                   this ABI
+ *   -----------------------------------------------  -------------------------------------------
+ *   call petsc_start      (mo_petsc.f90:41-128,       orctm_nh_create()
+ *                          called orctm.f90:970)
+ *   compute_matrix        (mo_petsc.f90:543-782)      done once inside orctm_nh_create()
+ *   call petsc_solve_3D   (mo_petsc.f90:211-233,      orctm_nh_solve()
+ *                          called ocvtot_nh.f90:150)
+ *   ComputeRHS            (mo_petsc.f90:237-397)      done inside orctm_nh_solve()/orctm_nh_set_rhs()
+ *   call petsc_end        (mo_petsc.f90:131-143)      orctm_nh_destroy()
+ *   MatMult (PETSc AIJ, used inside KSPSolve)        orctm_nh_apply()
+ *
+ * All arrays crossing the boundary are the caller's Fortran arrays, column-major,
+ * including the one-cell halo ring of ORCTM's local arrays (mo_commo1.f90:105-146).
+ * Every entry point returns an int status (ORCTM_NH_OK == 0) and never aborts.
+ * No HIP or torch types appear in the signatures; one handle = one system on one GPU.
+ */
+#ifndef ORCTM_NH_H
+#define ORCTM_NH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORCTM_NH_ABI_VERSION 1
+
+/* Status codes. The reference ignores PETSc's ierr and never queries the converged
+ * reason (mo_petsc.f90:219, no KSPGetConvergedReason); here every outcome is reported. */
+enum orctm_nh_status {
+  ORCTM_NH_OK = 0,              /* converged to rtol */
+  ORCTM_NH_NOT_CONVERGED = 1,   /* maxit / max_outer reached; pnh holds the last iterate */
+  ORCTM_NH_BREAKDOWN = 2,       /* Krylov breakdown (p.Ap <= 0 or r.z <= 0 or non-finite scalar) */
+  ORCTM_NH_ERR_NONFINITE = 3,   /* NaN/Inf in the right-hand side; pnh left untouched */
+  ORCTM_NH_ERR_INVALID = 4,     /* invalid geometry or arguments */
+  ORCTM_NH_ERR_NOMEM = 5,       /* device allocation failed or admission refused */
+  ORCTM_NH_ERR_HIP = 6,         /* HIP runtime error */
+  ORCTM_NH_ERR_NODEVICE = 7     /* no usable gfx950 device */
+};
+
+/* Precision modes. */
+enum orctm_nh_precision {
+  ORCTM_NH_FP64 = 0,   /* FP64 operator, FP64 PCG */
+  ORCTM_NH_MIXED = 1   /* FP32 operator copy + FP32 PCG (FP32 reductions) inside FP64 iterative refinement */
+};
+
+/* Operator inputs. Field meanings follow the reference exactly:
+ *   ie, je, ke  : IE, JE, KE of mo_param1.f90 -- halo'd local sizes. The Poisson unknowns are the
+ *                 interior (IE-2) x (JE-2) x KE cells, the DMDA of mo_petsc.f90:48-50,75-95.
+ *   i_start     : lower bound of u-point arrays, DLXU(I_start:IE,JE) (mo_commo1.f90:105; 0 on the
+ *                 west-most rank, mo_parallel.f90:197-201).
+ *   j_start     : lower bound of v-point arrays, DLYV(IE,J_start:JE) (mo_commo1.f90:106).
+ *   dlxp, dlyp  : DLXP(IE,JE), DLYP(IE,JE)  grid spacings at p-points  [m]
+ *   dlxu        : DLXU(I_start:IE,JE)        x spacing at u-points      [m]
+ *   dlyv        : DLYV(IE,J_start:JE)        y spacing at v-points      [m]
+ *   ddpo        : DDPO(IE,JE,KE)  p-cell thickness incl. partial bottom cells, 0 below (boden.f90:186-190)
+ *   weto        : WETO(IE,JE,KE)  land/sea mask, 1 = wet (boden.f90:185)
+ *   dzw         : DZW(KE)         layer thicknesses (beleg.f90:87-105)
+ *   cyclic_x/y  : CYCLICWE / CYCLICNS (mo_param1.f90:62-77; periodic DMDA, mo_petsc.f90:75-86)
+ *   open_w/e/n/s: MO_OBCS West/East/North/South (mo_obcs.f90:9-10); j grows southward, north = J-1.
+ * The library copies what it needs during orctm_nh_create(); the pointers are not kept. */
+typedef struct orctm_nh_grid {
+  int32_t ie, je, ke;
+  int32_t i_start, j_start;
+  const double* dlxp;
+  const double* dlyp;
+  const double* dlxu;
+  const double* dlyv;
+  const double* ddpo;
+  const double* weto;
+  const double* dzw;
+  int32_t cyclic_x, cyclic_y;
+  int32_t open_w, open_e, open_n, open_s;
+} orctm_nh_grid_t;
+
+/* Solver options. orctm_nh_default_options() fills the defaults listed here. */
+typedef struct orctm_nh_options {
+  int32_t precision;    /* ORCTM_NH_MIXED (default) or ORCTM_NH_FP64 */
+  int32_t maxit;        /* cap on total inner Krylov iterations per solve (default 10000, the KSP default) */
+  int32_t max_outer;    /* cap on FP64 refinement steps per solve (default 40) */
+  int32_t warm_start;   /* 1 (default): after the first solve, start from the previous solution,
+                           as KSPSetInitialGuessNonzero(ksp,PETSC_TRUE) does (mo_petsc.f90:232) */
+  double rtol;          /* outer stop: ||b - A p||_2 <= rtol ||b||_2, FP64, reference operator (default 1e-10) */
+  double rtol_inner;    /* inner relative reduction per refinement step (default 1e-5 MIXED, 0 = as needed FP64) */
+  int32_t check_every;  /* inner iterations per device-side chunk between host polls (default 16) */
+  int32_t device;       /* HIP device ordinal (default 0) */
+  int32_t profile;      /* 1: time the inner kernels with HIP events (orctm_nh_profile_read); default 0 */
+  int32_t reserved[5];
+} orctm_nh_options_t;
+
+/* Per-solve statistics. */
+typedef struct orctm_nh_stats {
+  int32_t status;       /* enum orctm_nh_status of this solve */
+  int32_t outer_iters;  /* FP64 refinement steps taken */
+  int32_t inner_iters;  /* total inner PCG iterations */
+  int32_t reserved0;
+  double rel_res;       /* final ||b - A p||_2 / ||b||_2 in FP64 with the reference operator */
+  double bnorm;         /* ||b||_2 */
+  double solve_ms;      /* device time of the solve (HIP events), excluding host<->device copies */
+  double total_ms;      /* host wall time of the call, including copies */
+  double reserved[4];
+} orctm_nh_stats_t;
+
+typedef struct orctm_nh orctm_nh_t;
+
+/* Library and device information. */
+int orctm_nh_abi_version(void);
+int orctm_nh_device_count(int32_t* n);
+void orctm_nh_default_options(orctm_nh_options_t* opts);
+const char* orctm_nh_status_string(int status);
+/* Text of the last HIP failure seen by the calling thread (empty if none). */
+const char* orctm_nh_last_error(void);
+
+/* Replaces petsc_start (mo_petsc.f90:41-128) + compute_matrix (mo_petsc.f90:543-782):
+ * validates the grid, uploads the metric/mask arrays and assembles the operator on the device,
+ * once, as KSPSetUp assembles it once. opts may be NULL (defaults). */
+int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts, orctm_nh_t** handle);
+
+/* Replaces petsc_solve_3D (mo_petsc.f90:211-233) with its ComputeRHS callback (mo_petsc.f90:376):
+ * reads DIVG(IE,JE,KE) (interior x WETO is the right-hand side), solves A p = b and writes the
+ * interior of PNH(IE,JE,KE) only (mo_petsc.f90:223-227); the halo of pnh is not touched.
+ * Initial guess: the handle's previous solution when warm_start is set (KSPSetInitialGuessNonzero),
+ * zero on the first call. stats may be NULL. */
+int orctm_nh_solve(orctm_nh_t* h, const double* divg, double* pnh, orctm_nh_stats_t* stats);
+
+/* Replaces petsc_end (mo_petsc.f90:131-143). Frees all device memory of the handle. */
+int orctm_nh_destroy(orctm_nh_t* h);
+
+/* Split form of orctm_nh_solve for callers that keep data resident on the device
+ * (serving loops, benchmarks):  set_rhs -> solve_resident -> get_solution. */
+int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg);            /* ComputeRHS, mo_petsc.f90:376 */
+int orctm_nh_set_guess(orctm_nh_t* h, const double* pnh);           /* PNH interior as initial guess */
+int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* stats); /* KSPSolve on resident b, p */
+int orctm_nh_get_solution(orctm_nh_t* h, double* pnh);              /* mo_petsc.f90:223-227 */
+
+/* y = A x with the reference operator in FP64 (the AIJ MatMult that KSPSolve applies to the matrix
+ * of compute_matrix). x, y: interior vectors, (IE-2)*(JE-2)*KE doubles in DMDA natural order
+ * (i fastest, then j, then k) -- the order of res_array in mo_petsc.f90:222-227. */
+int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y);
+
+/* Reads back the assembled reference operator: 7 planes of (IE-2)*(JE-2)*KE doubles, in the order
+ * centre, west(i-1), east(i+1), north(j-1), south(j+1), up(k-1), down(k+1); absent couplings are 0.
+ * These are the MatSetValuesStencil values of mo_petsc.f90:623-765. */
+int orctm_nh_get_operator(orctm_nh_t* h, double* coef7);
+
+/* Size of the interior: nx = IE-2, ny = JE-2, nz = KE. */
+int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz);
+
+/* Bytes of device memory the handle holds. */
+int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes);
+
+/* Device time of the two inner-iteration kernels, accumulated over the launches that did work
+ * (HIP events recorded on the solver's own stream, inside the captured iteration graph). Needs
+ * options.profile = 1. No counterpart in the reference (PETSc's -log_view is never enabled). */
+typedef struct orctm_nh_profile {
+  double matvec_ms;         /* fused search-direction update + 7-point matvec + p.q */
+  double update_ms;         /* x/r update + vertical-line solve + r.z, r.r */
+  int64_t matvec_launches;
+  int64_t update_launches;
+  int64_t cells;            /* interior cells covered by one launch (all systems of the context) */
+  int64_t reserved0;
+  double reserved[2];
+} orctm_nh_profile_t;
+int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* prof, int32_t reset);
+
+/* ------------------------------------------------------------------------------------------
+ * Building blocks of the inner solve, exposed for verification. No counterpart in the reference
+ * (its Krylov internals live inside PETSc). "inner" is the scaled operator C = S (-A),
+ * S = DLXP*DLYP/ZP per cell, held in FP32 (MIXED) or FP64 (FP64); x, y, r, z are interior vectors
+ * in natural order of that precision (float or double). */
+int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y);
+/* z = M^{-1} r with M the vertical-line (per-column tridiagonal) preconditioner of C;
+ * *rho receives r.z as accumulated by the device kernel (may be NULL). */
+int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched solves: many independent systems (ensemble members, nests) of uneven sizes advanced
+ * together, one launch per solver phase for the whole batch, with per-system convergence masks.
+ * The handles keep ownership of their device data; a batch only references them. */
+typedef struct orctm_nh_batch orctm_nh_batch_t;
+int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_t** batch);
+/* Solves every member on its resident b (orctm_nh_set_rhs) and guess; stats has n entries (may be NULL).
+ * Returns ORCTM_NH_OK when every member converged, else the first non-OK member status. */
+int orctm_nh_batch_solve(orctm_nh_batch_t* batch, orctm_nh_stats_t* stats);
+int orctm_nh_batch_destroy(orctm_nh_batch_t* batch);
+int orctm_nh_batch_profile_read(orctm_nh_batch_t* batch, orctm_nh_profile_t* prof, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORCTM_NH_H */

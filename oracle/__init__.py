@@ -1,0 +1,199 @@
+"""CPU oracle for the nonhydrostatic pressure Poisson path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
+package, and only to check (or time beside) the HIP library; the product path never does.
+The arithmetic lives in ``nh_oracle.c`` (built to ``liboracle_nh.so`` by ``oracle/Makefile``); each
+function there cites the reference file:line it restates. This module only marshals numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_nh.so")
+NC = 7
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        from orctm_amd.abi import GridC, StatsC
+
+        i, sz, vp = ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p
+        L.orc_assemble.argtypes = [ctypes.POINTER(GridC), _dp, _dp]
+        L.orc_rhs.argtypes = [ctypes.POINTER(GridC), _dp, _dp]
+        L.orc_scatter_interior.argtypes = [ctypes.POINTER(GridC), _dp, _dp]
+        L.orc_matvec64.argtypes = [i, i, i, i, i, _dp, _dp, _dp]
+        L.orc_matvec32.argtypes = [i, i, i, i, i, _fp, _fp, _fp]
+        L.orc_inner_coef64.argtypes = [sz, _dp, _dp, _dp]
+        L.orc_inner_coef32.argtypes = [sz, _dp, _dp, _fp]
+        L.orc_line_factor64.argtypes = [i, i, i, _dp, _dp, _dp]
+        L.orc_line_factor32.argtypes = [i, i, i, _fp, _fp, _fp]
+        L.orc_line_apply64.argtypes = [i, i, i, _dp, _dp, _dp, _dp, _dp]
+        L.orc_line_apply32.argtypes = [i, i, i, _fp, _fp, _fp, _fp, _dp]
+        L.orc_pcg64.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, i,
+                                ctypes.POINTER(ctypes.c_int)]
+        L.orc_pcg32.argtypes = [i, i, i, i, i, _fp, _fp, _fp, _fp, _fp, _fp, ctypes.c_float, i,
+                                ctypes.POINTER(ctypes.c_int)]
+        L.orc_solve.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp, i, ctypes.c_double, ctypes.c_double, i, i,
+                                ctypes.POINTER(StatsC)]
+        L.orc_fgmres.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp, i, ctypes.c_double, i, ctypes.POINTER(StatsC)]
+        L.orc_num_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"])
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    assert a.dtype == np.float32
+    return a.ctypes.data_as(_fp)
+
+
+def assemble(grid):
+    """(A[7,N], S[N]) of compute_matrix (mo_petsc.f90:543-782) for a synth.Grid."""
+    N = grid.cells
+    A = np.zeros((NC, N))
+    S = np.zeros(N)
+    rc = lib().orc_assemble(ctypes.byref(grid.c_struct()), _d(A), _d(S))
+    if rc != 0:
+        raise ValueError(f"orc_assemble status {rc}")
+    return A, S
+
+
+def rhs(grid, divg):
+    """b = DIVG*WETO on the interior (mo_petsc.f90:376), natural order."""
+    b = np.zeros(grid.cells)
+    divg = np.asfortranarray(divg, dtype=np.float64)
+    lib().orc_rhs(ctypes.byref(grid.c_struct()), _d(divg), _d(b))
+    return b
+
+
+def cyc(grid):
+    return int(grid.spec.cyclic_x), int(grid.spec.cyclic_y)
+
+
+def matvec(grid, A, x):
+    A = np.ascontiguousarray(A)
+    x = np.ascontiguousarray(x)
+    if A.dtype == np.float32:
+        y = np.zeros(grid.cells, dtype=np.float32)
+        lib().orc_matvec32(grid.nx, grid.ny, grid.nz, *cyc(grid), _f(A), _f(x.astype(np.float32)), _f(y))
+    else:
+        y = np.zeros(grid.cells)
+        lib().orc_matvec64(grid.nx, grid.ny, grid.nz, *cyc(grid), _d(A), _d(x), _d(y))
+    return y
+
+
+def inner_coef(A, S, dtype=np.float64):
+    N = S.size
+    if dtype == np.float32:
+        C = np.zeros((NC, N), dtype=np.float32)
+        lib().orc_inner_coef32(N, _d(np.ascontiguousarray(A)), _d(S), _f(C))
+    else:
+        C = np.zeros((NC, N))
+        lib().orc_inner_coef64(N, _d(np.ascontiguousarray(A)), _d(S), _d(C))
+    return C
+
+
+def line_factor(grid, C):
+    invD = np.zeros(grid.cells, dtype=C.dtype)
+    L = np.zeros(grid.cells, dtype=C.dtype)
+    if C.dtype == np.float32:
+        lib().orc_line_factor32(grid.nx, grid.ny, grid.nz, _f(C), _f(invD), _f(L))
+    else:
+        lib().orc_line_factor64(grid.nx, grid.ny, grid.nz, _d(C), _d(invD), _d(L))
+    return invD, L
+
+
+def line_apply(grid, invD, L, r):
+    z = np.zeros(grid.cells, dtype=invD.dtype)
+    rho = ctypes.c_double(0.0)
+    r = np.ascontiguousarray(r, dtype=invD.dtype)
+    if invD.dtype == np.float32:
+        lib().orc_line_apply32(grid.nx, grid.ny, grid.nz, _f(invD), _f(L), _f(r), _f(z), ctypes.byref(rho))
+    else:
+        lib().orc_line_apply64(grid.nx, grid.ny, grid.nz, _d(invD), _d(L), _d(r), _d(z), ctypes.byref(rho))
+    return z, rho.value
+
+
+def solve(grid, A, S, b, precision=0, rtol=1e-10, rtol_inner=None, maxit=10000, max_outer=40, x0=None):
+    """FP64 refinement around the inner PCG (the HIP library's algorithm) on the CPU."""
+    from orctm_amd.abi import StatsC
+
+    if rtol_inner is None:
+        rtol_inner = 1e-5 if precision == 1 else 0.0
+    p = np.zeros(grid.cells) if x0 is None else np.array(x0, dtype=np.float64)
+    st = StatsC()
+    lib().orc_solve(grid.nx, grid.ny, grid.nz, *cyc(grid), _d(np.ascontiguousarray(A)), _d(S),
+                    _d(np.ascontiguousarray(b)), _d(p), precision, rtol, rtol_inner, maxit, max_outer,
+                    ctypes.byref(st))
+    return p, st
+
+
+def fgmres(grid, A, S, b, restart=30, rtol=1e-10, maxit=20000, x0=None):
+    """FGMRES(restart), the reference's KSP type, FP64, line-preconditioned."""
+    from orctm_amd.abi import StatsC
+
+    x = np.zeros(grid.cells) if x0 is None else np.array(x0, dtype=np.float64)
+    st = StatsC()
+    lib().orc_fgmres(grid.nx, grid.ny, grid.nz, *cyc(grid), _d(np.ascontiguousarray(A)), _d(S),
+                     _d(np.ascontiguousarray(b)), _d(x), restart, rtol, maxit, ctypes.byref(st))
+    return x, st
+
+
+def rel_res(grid, A, p, b):
+    r = b - matvec(grid, A, p)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(r) / nb) if nb > 0 else float(np.linalg.norm(r))
+
+
+def to_scipy(grid, A):
+    """Sparse CSR matrix of the assembled operator (for direct solves in tests)."""
+    import scipy.sparse as sp
+
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    N = grid.cells
+    n = np.arange(N)
+    i = n % nx
+    j = (n // nx) % ny
+    k = n // (nx * ny)
+    cx, cy = cyc(grid)
+    rows, cols, vals = [n], [n], [A[0]]
+    nbrs = [
+        (1, np.where(i > 0, n - 1, np.where(cx, n + nx - 1, -1))),
+        (2, np.where(i < nx - 1, n + 1, np.where(cx, n - (nx - 1), -1))),
+        (3, np.where(j > 0, n - nx, np.where(cy, n + nx * (ny - 1), -1))),
+        (4, np.where(j < ny - 1, n + nx, np.where(cy, n - nx * (ny - 1), -1))),
+        (5, np.where(k > 0, n - nx * ny, -1)),
+        (6, np.where(k < nz - 1, n + nx * ny, -1)),
+    ]
+    for q, c in nbrs:
+        m = (c >= 0) & (A[q] != 0)
+        rows.append(n[m])
+        cols.append(c[m])
+        vals.append(A[q][m])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,126 @@
+"""ctypes mirror of include/orctm_nh.h and the loader of the in-tree HIP library.
+
+The library (``orctm_amd/liborctm_nh.so``) is built by ``orctm_amd/build.py`` with hipcc for
+gfx950. Loading fails loudly if it is missing: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborctm_nh.so")
+
+OK, NOT_CONVERGED, BREAKDOWN, ERR_NONFINITE, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE = range(8)
+STATUS_NAMES = ["OK", "NOT_CONVERGED", "BREAKDOWN", "ERR_NONFINITE", "ERR_INVALID", "ERR_NOMEM", "ERR_HIP",
+                "ERR_NODEVICE"]
+FP64, MIXED = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class GridC(ctypes.Structure):
+    _fields_ = [
+        ("ie", ctypes.c_int32), ("je", ctypes.c_int32), ("ke", ctypes.c_int32),
+        ("i_start", ctypes.c_int32), ("j_start", ctypes.c_int32),
+        ("dlxp", _dp), ("dlyp", _dp), ("dlxu", _dp), ("dlyv", _dp), ("ddpo", _dp), ("weto", _dp), ("dzw", _dp),
+        ("cyclic_x", ctypes.c_int32), ("cyclic_y", ctypes.c_int32),
+        ("open_w", ctypes.c_int32), ("open_e", ctypes.c_int32), ("open_n", ctypes.c_int32),
+        ("open_s", ctypes.c_int32),
+    ]
+
+
+class OptionsC(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32), ("maxit", ctypes.c_int32), ("max_outer", ctypes.c_int32),
+        ("warm_start", ctypes.c_int32), ("rtol", ctypes.c_double), ("rtol_inner", ctypes.c_double),
+        ("check_every", ctypes.c_int32), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class ProfileC(ctypes.Structure):
+    _fields_ = [
+        ("matvec_ms", ctypes.c_double), ("update_ms", ctypes.c_double), ("matvec_launches", ctypes.c_int64),
+        ("update_launches", ctypes.c_int64), ("cells", ctypes.c_int64), ("reserved0", ctypes.c_int64),
+        ("reserved", ctypes.c_double * 2),
+    ]
+
+
+class StatsC(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("outer_iters", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("rel_res", ctypes.c_double), ("bnorm", ctypes.c_double),
+        ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("reserved", ctypes.c_double * 4),
+    ]
+
+    def as_dict(self):
+        return {"status": STATUS_NAMES[self.status] if 0 <= self.status < len(STATUS_NAMES) else self.status,
+                "outer_iters": self.outer_iters, "inner_iters": self.inner_iters, "rel_res": self.rel_res,
+                "bnorm": self.bnorm, "solve_ms": self.solve_ms, "total_ms": self.total_ms}
+
+
+# Every symbol include/orctm_nh.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "orctm_nh_abi_version", "orctm_nh_device_count", "orctm_nh_default_options", "orctm_nh_status_string",
+    "orctm_nh_last_error",
+    "orctm_nh_create", "orctm_nh_solve", "orctm_nh_destroy", "orctm_nh_set_rhs", "orctm_nh_set_guess",
+    "orctm_nh_solve_resident", "orctm_nh_get_solution", "orctm_nh_apply", "orctm_nh_get_operator",
+    "orctm_nh_get_dims", "orctm_nh_device_bytes", "orctm_nh_inner_apply", "orctm_nh_inner_precond",
+    "orctm_nh_batch_create", "orctm_nh_batch_solve", "orctm_nh_batch_destroy", "orctm_nh_profile_read",
+    "orctm_nh_batch_profile_read",
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load liborctm_nh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m orctm_amd.build` "
+                           "(hipcc, gfx950). There is no CPU fallback.")
+    lib = ctypes.CDLL(path)
+    vp, ip = ctypes.c_void_p, ctypes.c_int
+    lib.orctm_nh_abi_version.restype = ip
+    lib.orctm_nh_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+    lib.orctm_nh_default_options.argtypes = [ctypes.POINTER(OptionsC)]
+    lib.orctm_nh_default_options.restype = None
+    lib.orctm_nh_status_string.argtypes = [ip]
+    lib.orctm_nh_status_string.restype = ctypes.c_char_p
+    lib.orctm_nh_last_error.argtypes = []
+    lib.orctm_nh_last_error.restype = ctypes.c_char_p
+    lib.orctm_nh_create.argtypes = [ctypes.POINTER(GridC), ctypes.POINTER(OptionsC), ctypes.POINTER(vp)]
+    lib.orctm_nh_solve.argtypes = [vp, _dp, _dp, ctypes.POINTER(StatsC)]
+    lib.orctm_nh_destroy.argtypes = [vp]
+    lib.orctm_nh_set_rhs.argtypes = [vp, _dp]
+    lib.orctm_nh_set_guess.argtypes = [vp, _dp]
+    lib.orctm_nh_solve_resident.argtypes = [vp, ctypes.POINTER(StatsC)]
+    lib.orctm_nh_get_solution.argtypes = [vp, _dp]
+    lib.orctm_nh_apply.argtypes = [vp, _dp, _dp]
+    lib.orctm_nh_get_operator.argtypes = [vp, _dp]
+    lib.orctm_nh_get_dims.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int32)] * 3
+    lib.orctm_nh_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.orctm_nh_inner_apply.argtypes = [vp, vp, vp]
+    lib.orctm_nh_inner_precond.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+    lib.orctm_nh_batch_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    lib.orctm_nh_batch_solve.argtypes = [vp, ctypes.POINTER(StatsC)]
+    lib.orctm_nh_batch_destroy.argtypes = [vp]
+    lib.orctm_nh_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
+    lib.orctm_nh_batch_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
+    for name in EXPORTS:
+        f = getattr(lib, name)
+        if f.restype is ctypes.c_int and name not in ("orctm_nh_default_options",):
+            f.restype = ip
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().orctm_nh_last_error().decode(errors="replace")
+
+
+def status_name(code: int) -> str:
+    return STATUS_NAMES[code] if 0 <= code < len(STATUS_NAMES) else f"status {code}"

@@ -1,0 +1,846 @@
+// nh_api.hip -- C ABI of the gfx950 nonhydrostatic pressure Poisson solver (include/orctm_nh.h).
+//
+// A handle owns one system's device data: the assembled reference operator (FP64), the scaled
+// inner operator and line-preconditioner factors (FP32 or FP64), the right-hand side, the FP64
+// solution kept resident for warm starts (KSPSetInitialGuessNonzero, mo_petsc.f90:232) and the
+// inner work vectors. A solver context (Ctx) drives one or more systems through the same kernel
+// launches: FP64 residual -> inner PCG in fixed-length chunks captured in a hipGraph (convergence
+// polled between chunks) -> FP64 refinement update, until every system has converged.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/orctm_nh.h"
+#include "nh_kernels.h"
+
+using namespace onh;
+
+namespace {
+
+struct Ctx {
+  int nsys = 0;
+  bool fp32 = true;
+  int nzr = 0;
+  int ntiles = 0;
+  int max_outer = 0;
+  int maxit = 10000;
+  int check_every = 16;
+  int device = 0;
+  Sys* d_sys = nullptr;
+  Tile* d_tiles = nullptr;
+  Outer* d_outer = nullptr;
+  void* d_inner = nullptr;
+  Outer* h_outer = nullptr;  // pinned
+  void* h_inner = nullptr;   // pinned
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // optional kernel timing: events around every matvec / update launch inside the graph
+  bool prof = false;
+  std::vector<hipEvent_t> kev;      // 3 per captured iteration
+  std::vector<int> iter_seen;       // per system, inner iterations already accounted
+  double mv_ms = 0.0, up_ms = 0.0;
+  long long mv_n = 0, up_n = 0, cells = 0;
+};
+
+size_t inner_size(bool fp32) { return fp32 ? sizeof(Inner<float>) : sizeof(Inner<double>); }
+
+int inner_iter(const Ctx& c, int q) {
+  if (c.fp32) return reinterpret_cast<const Inner<float>*>(c.h_inner)[q].iter;
+  return reinterpret_cast<const Inner<double>*>(c.h_inner)[q].iter;
+}
+
+int inner_done(const Ctx& c, int q) {
+  if (c.fp32) return reinterpret_cast<const Inner<float>*>(c.h_inner)[q].done;
+  return reinterpret_cast<const Inner<double>*>(c.h_inner)[q].done;
+}
+
+}  // namespace
+
+struct orctm_nh {
+  orctm_nh_options_t opts;
+  int IE = 0, JE = 0, KE = 0, nx = 0, ny = 0, nz = 0, pitch = 0;
+  long long plane = 0;
+  bool fp32 = true;
+  bool guess_set = false;
+  Sys sys{};  // host copy, pointers into device memory
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  double* d_fort = nullptr;        // staging for a Fortran (IE,JE,KE) array
+  double* d_nb = nullptr;          // ||b|| written by k_rhs
+  unsigned long long* d_smax = nullptr;
+  Ctx ctx;                         // single-system context
+  std::vector<Tile> tiles;         // this system's tiles (local order)
+  std::mutex mu;
+};
+
+struct orctm_nh_batch {
+  std::vector<orctm_nh_t*> members;
+  Ctx ctx;
+};
+
+namespace {
+
+thread_local char g_last_error[256] = "";
+
+hipError_t note_error(hipError_t e, const char* what, int line) {
+  if (e != hipSuccess)
+    snprintf(g_last_error, sizeof(g_last_error), "%s (%d) at nh_api.hip:%d: %s", hipGetErrorName(e), (int)e, line,
+             what);
+  return e;
+}
+
+#define HIPCHK(x)                                                   \
+  do {                                                              \
+    hipError_t e_ = (x);                                            \
+    if (e_ != hipSuccess) return note_error(e_, #x, __LINE__);      \
+  } while (0)
+
+int map_err(hipError_t e) {
+  if (e == hipSuccess) return ORCTM_NH_OK;
+  if (g_last_error[0] == 0) note_error(e, "device call", 0);
+  if (e == hipErrorOutOfMemory) return ORCTM_NH_ERR_NOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice || e == hipErrorNoBinaryForGpu) return ORCTM_NH_ERR_NODEVICE;
+  return ORCTM_NH_ERR_HIP;
+}
+
+hipError_t dalloc(orctm_nh_t* h, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return e;
+  h->allocs.push_back(*p);
+  h->bytes += bytes;
+  return hipMemset(*p, 0, bytes);
+}
+
+void free_all(orctm_nh_t* h) {
+  for (void* p : h->allocs) hipFree(p);
+  h->allocs.clear();
+}
+
+// Tiles of one system in natural (row-major) order.
+std::vector<Tile> system_tiles(int sys_index, int nx, int ny) {
+  std::vector<Tile> t;
+  const int tx = (nx + TX - 1) / TX, ty = (ny + TY - 1) / TY;
+  t.reserve((size_t)tx * ty);
+  int local = 0;
+  for (int by = 0; by < ty; ++by)
+    for (int bx = 0; bx < tx; ++bx) t.push_back(Tile{sys_index, bx, by, local++});
+  return t;
+}
+
+// Block b runs on XCD b % 8 (round-robin dispatch); give each XCD a contiguous run of tiles so
+// that vertically and horizontally adjacent tiles share an L2.
+std::vector<Tile> xcd_order(const std::vector<Tile>& nat) {
+  const int T = (int)nat.size(), X = 8;
+  std::vector<Tile> out(T);
+  std::vector<int> start(X + 1, 0);
+  for (int x = 0; x < X; ++x) start[x + 1] = start[x] + (T > x ? (T - x + X - 1) / X : 0);
+  for (int b = 0; b < T; ++b) out[b] = nat[start[b % X] + b / X];
+  return out;
+}
+
+void ctx_destroy(Ctx& c) {
+  if (c.graph) hipGraphExecDestroy(c.graph);
+  if (c.d_sys) hipFree(c.d_sys);
+  if (c.d_tiles) hipFree(c.d_tiles);
+  if (c.d_outer) hipFree(c.d_outer);
+  if (c.d_inner) hipFree(c.d_inner);
+  if (c.h_outer) hipHostFree(c.h_outer);
+  if (c.h_inner) hipHostFree(c.h_inner);
+  if (c.ev0) hipEventDestroy(c.ev0);
+  if (c.ev1) hipEventDestroy(c.ev1);
+  for (hipEvent_t e : c.kev) hipEventDestroy(e);
+  if (c.stream) hipStreamDestroy(c.stream);
+  c = Ctx{};
+}
+
+// Builds a context over the given systems (host Sys copies; outer/inner pointers are replaced by
+// the context's own contiguous arrays).
+hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<std::vector<Tile>>& tiles, bool fp32,
+                     int check_every, int max_outer, int device, bool prof) {
+  c.nsys = (int)systems.size();
+  c.prof = prof;
+  c.iter_seen.assign(c.nsys, 0);
+  for (const Sys& s : systems) c.cells += (long long)s.nx * s.ny * s.nz;
+  c.fp32 = fp32;
+  c.check_every = check_every > 0 ? check_every : 16;
+  c.max_outer = max_outer;
+  c.device = device;
+  for (const Sys& s : systems) c.maxit = std::max(c.maxit, s.maxit);
+  int maxnz = 1;
+  std::vector<Tile> nat;
+  for (int q = 0; q < c.nsys; ++q) {
+    maxnz = std::max(maxnz, systems[q].nz);
+    for (Tile t : tiles[q]) {
+      t.sys = q;
+      nat.push_back(t);
+    }
+  }
+  c.nzr = pick_nzr(fp32, maxnz);
+  std::vector<Tile> ord = xcd_order(nat);
+  c.ntiles = (int)ord.size();
+  HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  if (c.prof) {
+    c.kev.assign(3 * (size_t)c.check_every, nullptr);
+    for (hipEvent_t& e : c.kev) HIPCHK(hipEventCreate(&e));
+  }
+  HIPCHK(hipEventCreate(&c.ev0));
+  HIPCHK(hipEventCreate(&c.ev1));
+  HIPCHK(hipMalloc(&c.d_outer, sizeof(Outer) * c.nsys));
+  HIPCHK(hipMemset(c.d_outer, 0, sizeof(Outer) * c.nsys));
+  HIPCHK(hipMalloc(&c.d_inner, inner_size(fp32) * c.nsys));
+  HIPCHK(hipMemset(c.d_inner, 0, inner_size(fp32) * c.nsys));
+  HIPCHK(hipHostMalloc((void**)&c.h_outer, sizeof(Outer) * c.nsys, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&c.h_inner, inner_size(fp32) * c.nsys, hipHostMallocDefault));
+  std::vector<Sys> dev = systems;
+  for (int q = 0; q < c.nsys; ++q) {
+    dev[q].outer = c.d_outer + q;
+    dev[q].inner = static_cast<char*>(c.d_inner) + inner_size(fp32) * q;
+  }
+  HIPCHK(hipMalloc(&c.d_sys, sizeof(Sys) * c.nsys));
+  HIPCHK(hipMemcpy(c.d_sys, dev.data(), sizeof(Sys) * c.nsys, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&c.d_tiles, sizeof(Tile) * std::max(c.ntiles, 1)));
+  if (c.ntiles) HIPCHK(hipMemcpy(c.d_tiles, ord.data(), sizeof(Tile) * c.ntiles, hipMemcpyHostToDevice));
+  return hipSuccess;
+}
+
+// check_every iterations (matvec + column update) captured once, replayed per chunk. With kernel
+// timing on, the same launches are issued directly with HIP events around each kernel instead.
+hipError_t ctx_chunk(Ctx& c) {
+  if (c.prof) {
+    for (int it = 0; it < c.check_every; ++it) {
+      HIPCHK(hipEventRecord(c.kev[3 * it], c.stream));
+      HIPCHK(launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+      HIPCHK(hipEventRecord(c.kev[3 * it + 1], c.stream));
+      HIPCHK(launch_update(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles));
+      HIPCHK(hipEventRecord(c.kev[3 * it + 2], c.stream));
+    }
+    return hipSuccess;
+  }
+  if (!c.graph) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < c.check_every && e == hipSuccess; ++it) {
+      e = launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles);
+      if (e == hipSuccess) e = launch_update(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles);
+    }
+    hipError_t e2 = hipStreamEndCapture(c.stream, &g);
+    if (e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return e;
+    }
+    HIPCHK(e2);
+    e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(e);
+  }
+  return hipGraphLaunch(c.graph, c.stream);
+}
+
+// Runs the whole refinement for every system of the context; results stay on the device.
+hipError_t ctx_solve(Ctx& c, float* ms) {
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  HIPCHK(launch_begin(c.stream, c.fp32, c.d_sys, c.nsys));
+  HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+  for (int m = 0; m <= c.max_outer + 1; ++m) {
+    HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    bool any = false;
+    for (int q = 0; q < c.nsys; ++q) any |= (c.h_outer[q].odone == RUN);
+    if (!any) break;
+    HIPCHK(launch_init(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles));
+    std::fill(c.iter_seen.begin(), c.iter_seen.end(), 0);
+    // every inner solve ends within maxit iterations (device-side NOTCONV); the host bound below
+    // only guarantees termination of this loop whatever the device flags say
+    const int max_chunks = c.maxit / c.check_every + 2;
+    for (int chunk = 0; chunk < max_chunks; ++chunk) {
+      HIPCHK(ctx_chunk(c));
+      HIPCHK(hipMemcpyAsync(c.h_inner, c.d_inner, inner_size(c.fp32) * c.nsys, hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      bool run = false;
+      int worked = 0;
+      for (int q = 0; q < c.nsys; ++q) {
+        run |= (inner_done(c, q) == RUN);
+        const int it = inner_iter(c, q);
+        worked = std::max(worked, it - c.iter_seen[q]);
+        c.iter_seen[q] = it;
+      }
+      if (c.prof) {
+        for (int it = 0; it < std::min(worked, c.check_every); ++it) {
+          float a = 0.f, b = 0.f;
+          HIPCHK(hipEventElapsedTime(&a, c.kev[3 * it], c.kev[3 * it + 1]));
+          HIPCHK(hipEventElapsedTime(&b, c.kev[3 * it + 1], c.kev[3 * it + 2]));
+          c.mv_ms += a;
+          c.up_ms += b;
+          c.mv_n += 1;
+          c.up_n += 1;
+        }
+      }
+      if (!run) break;
+    }
+    HIPCHK(launch_refine(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+    HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+  }
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (ms) HIPCHK(hipEventElapsedTime(ms, c.ev0, c.ev1));
+  return hipSuccess;
+}
+
+int outer_status(const Outer& o) {
+  switch (o.odone) {
+    case CONVERGED: return ORCTM_NH_OK;
+    case NOTCONV: return ORCTM_NH_NOT_CONVERGED;
+    case NONFINITE: return ORCTM_NH_ERR_NONFINITE;
+    case RUN: return ORCTM_NH_NOT_CONVERGED;
+    default: return ORCTM_NH_BREAKDOWN;
+  }
+}
+
+void fill_stats(const Outer& o, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
+  if (!st) return;
+  memset(st, 0, sizeof(*st));
+  st->status = outer_status(o);
+  st->outer_iters = o.outer;
+  st->inner_iters = o.inner_total;
+  st->rel_res = o.rel;
+  st->bnorm = o.nb;
+  st->solve_ms = solve_ms;
+  st->total_ms = total_ms;
+}
+
+bool grid_valid(const orctm_nh_grid_t* g) {
+  if (!g) return false;
+  if (g->ie < 3 || g->je < 3 || g->ke < 1) return false;
+  if (g->i_start < 0 || g->i_start > 1 || g->j_start < 0 || g->j_start > 1) return false;
+  if (!g->dlxp || !g->dlyp || !g->dlxu || !g->dlyv || !g->ddpo || !g->weto || !g->dzw) return false;
+  const long long cells = (long long)(g->ie - 2) * (g->je - 2) * g->ke;
+  return cells > 0 && cells < (1LL << 40);
+}
+
+hipError_t upload(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }
+
+// Device part of create: allocations, assembly, factorisation, single-system context.
+int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
+  const size_t NP = (size_t)h->plane * h->nz;
+  const size_t es = h->fp32 ? sizeof(float) : sizeof(double);
+  Sys& s = h->sys;
+  memset(&s, 0, sizeof(s));
+  s.nx = h->nx;
+  s.ny = h->ny;
+  s.nz = h->nz;
+  s.pitch = h->pitch;
+  s.plane = h->plane;
+  s.cyc_x = g->cyclic_x ? 1 : 0;
+  s.cyc_y = g->cyclic_y ? 1 : 0;
+  s.tiles_x = (h->nx + TX - 1) / TX;
+  s.tiles_y = (h->ny + TY - 1) / TY;
+  s.ntiles = s.tiles_x * s.tiles_y;
+  void* p;
+  hipError_t e;
+#define ALLOC(field, bytes, type)              \
+  do {                                         \
+    e = dalloc(h, &p, (bytes));                \
+    if (e != hipSuccess) return map_err(e);    \
+    s.field = reinterpret_cast<type>(p);       \
+  } while (0)
+  ALLOC(A64, 7 * NP * sizeof(double), const double*);
+  ALLOC(S, NP * sizeof(double), const double*);
+  ALLOC(C, 7 * NP * es, const void*);
+  ALLOC(invD, NP * es, const void*);
+  ALLOC(L, NP * es, const void*);
+  ALLOC(wet, NP, const uint8_t*);
+  ALLOC(b, NP * sizeof(double), double*);
+  ALLOC(p64, NP * sizeof(double), double*);
+  ALLOC(x, NP * es, void*);
+  ALLOC(r, NP * es, void*);
+  ALLOC(z, NP * es, void*);
+  ALLOC(pa, NP * es, void*);
+  ALLOC(pb, NP * es, void*);
+  ALLOC(q, NP * es, void*);
+  ALLOC(partials, 2 * (size_t)s.ntiles * sizeof(double), double*);
+  ALLOC(counter, sizeof(unsigned), unsigned*);
+  e = dalloc(h, &p, (size_t)h->IE * h->JE * h->KE * sizeof(double));
+  if (e != hipSuccess) return map_err(e);
+  h->d_fort = static_cast<double*>(p);
+  e = dalloc(h, &p, sizeof(double));
+  if (e != hipSuccess) return map_err(e);
+  h->d_nb = static_cast<double*>(p);
+  e = dalloc(h, &p, sizeof(unsigned long long));
+  if (e != hipSuccess) return map_err(e);
+  h->d_smax = static_cast<unsigned long long*>(p);
+#undef ALLOC
+  s.nbp = h->d_nb;
+  s.smaxp = reinterpret_cast<const double*>(h->d_smax);
+  s.rtol = h->opts.rtol;
+  s.rtol_inner = h->opts.rtol_inner;
+  s.maxit = h->opts.maxit;
+  s.max_outer = h->opts.max_outer;
+
+  // temporary device copies of the Fortran inputs
+  const size_t n2 = (size_t)g->ie * g->je, n3 = n2 * g->ke;
+  const size_t nu = (size_t)(g->ie - g->i_start + 1) * g->je, nv = (size_t)g->ie * (g->je - g->j_start + 1);
+  double *d_dlxp = nullptr, *d_dlyp = nullptr, *d_dlxu = nullptr, *d_dlyv = nullptr, *d_ddpo = nullptr,
+         *d_weto = nullptr, *d_dzw = nullptr;
+  Sys* d_sys1 = nullptr;
+  Tile* d_tiles1 = nullptr;
+  int* d_bad = nullptr;
+  int bad = 0, rc = ORCTM_NH_OK;
+  std::vector<Tile> tl = system_tiles(0, h->nx, h->ny);
+  e = hipMalloc(&d_dlxp, n2 * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_dlyp, n2 * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_dlxu, nu * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_dlyv, nv * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_ddpo, n3 * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_weto, n3 * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_dzw, (size_t)g->ke * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_sys1, sizeof(Sys));
+  if (e == hipSuccess) e = hipMalloc(&d_tiles1, sizeof(Tile) * tl.size());
+  if (e == hipSuccess) e = hipMalloc(&d_bad, sizeof(int));
+  if (e == hipSuccess) e = upload(d_dlxp, g->dlxp, n2 * 8);
+  if (e == hipSuccess) e = upload(d_dlyp, g->dlyp, n2 * 8);
+  if (e == hipSuccess) e = upload(d_dlxu, g->dlxu, nu * 8);
+  if (e == hipSuccess) e = upload(d_dlyv, g->dlyv, nv * 8);
+  if (e == hipSuccess) e = upload(d_ddpo, g->ddpo, n3 * 8);
+  if (e == hipSuccess) e = upload(d_weto, g->weto, n3 * 8);
+  if (e == hipSuccess) e = upload(d_dzw, g->dzw, (size_t)g->ke * 8);
+  if (e == hipSuccess) e = upload(d_sys1, &s, sizeof(Sys));
+  if (e == hipSuccess) e = upload(d_tiles1, tl.data(), sizeof(Tile) * tl.size());
+  if (e == hipSuccess) e = hipMemset(d_bad, 0, sizeof(int));
+  if (e == hipSuccess) {
+    FortranIn f{};
+    f.IE = g->ie;
+    f.JE = g->je;
+    f.KE = g->ke;
+    f.i_start = g->i_start;
+    f.j_start = g->j_start;
+    f.cyc_x = g->cyclic_x ? 1 : 0;
+    f.cyc_y = g->cyclic_y ? 1 : 0;
+    f.open_w = g->open_w;
+    f.open_e = g->open_e;
+    f.open_n = g->open_n;
+    f.open_s = g->open_s;
+    f.dlxp = d_dlxp;
+    f.dlyp = d_dlyp;
+    f.dlxu = d_dlxu;
+    f.dlyv = d_dlyv;
+    f.ddpo = d_ddpo;
+    f.weto = d_weto;
+    f.dzw = d_dzw;
+    e = launch_assemble(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_bad, h->d_smax);
+    if (e == hipSuccess) e = launch_line_factor(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size());
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) rc = map_err(e);
+  else if (bad) rc = ORCTM_NH_ERR_INVALID;
+  hipFree(d_dlxp);
+  hipFree(d_dlyp);
+  hipFree(d_dlxu);
+  hipFree(d_dlyv);
+  hipFree(d_ddpo);
+  hipFree(d_weto);
+  hipFree(d_dzw);
+  hipFree(d_sys1);
+  hipFree(d_tiles1);
+  hipFree(d_bad);
+  if (rc != ORCTM_NH_OK) return rc;
+  h->tiles = tl;
+  e = ctx_build(h->ctx, {s}, {tl}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
+                h->opts.profile != 0);
+  return map_err(e);
+}
+
+// Padded device vector <-> natural-order host vector (elem bytes each).
+hipError_t to_device_vec(const orctm_nh_t* h, void* d, const void* host, size_t es) {
+  return hipMemcpy2D(d, (size_t)h->pitch * es, host, (size_t)h->nx * es, (size_t)h->nx * es,
+                     (size_t)h->ny * h->nz, hipMemcpyHostToDevice);
+}
+hipError_t to_host_vec(const orctm_nh_t* h, void* host, const void* d, size_t es) {
+  return hipMemcpy2D(host, (size_t)h->nx * es, d, (size_t)h->pitch * es, (size_t)h->nx * es, (size_t)h->ny * h->nz,
+                     hipMemcpyDeviceToHost);
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int prepare_guess(orctm_nh_t* h) {
+  // warm start: keep p64 (previous solution or set_guess); otherwise start from zero
+  if (!h->opts.warm_start && !h->guess_set) {
+    hipError_t e = hipMemsetAsync(h->sys.p64, 0, (size_t)h->plane * h->nz * sizeof(double), h->ctx.stream);
+    if (e != hipSuccess) return map_err(e);
+  }
+  h->guess_set = false;
+  return ORCTM_NH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orctm_nh_abi_version(void) { return ORCTM_NH_ABI_VERSION; }
+
+const char* orctm_nh_last_error(void) { return g_last_error; }
+
+int orctm_nh_device_count(int32_t* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (n) *n = (e == hipSuccess) ? c : 0;
+  return e == hipSuccess ? ORCTM_NH_OK : ORCTM_NH_ERR_NODEVICE;
+}
+
+void orctm_nh_default_options(orctm_nh_options_t* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->precision = ORCTM_NH_MIXED;
+  o->maxit = 10000;
+  o->max_outer = 40;
+  o->warm_start = 1;
+  o->rtol = 1e-10;
+  o->rtol_inner = -1.0;  // resolved at create: 1e-5 (MIXED) or 0 (FP64)
+  o->check_every = 16;
+  o->device = 0;
+}
+
+const char* orctm_nh_status_string(int s) {
+  switch (s) {
+    case ORCTM_NH_OK: return "converged";
+    case ORCTM_NH_NOT_CONVERGED: return "not converged (maxit/max_outer reached)";
+    case ORCTM_NH_BREAKDOWN: return "Krylov breakdown";
+    case ORCTM_NH_ERR_NONFINITE: return "non-finite right-hand side";
+    case ORCTM_NH_ERR_INVALID: return "invalid argument or geometry";
+    case ORCTM_NH_ERR_NOMEM: return "out of device memory";
+    case ORCTM_NH_ERR_HIP: return "HIP runtime error";
+    case ORCTM_NH_ERR_NODEVICE: return "no usable gfx950 device";
+    default: return "unknown status";
+  }
+}
+
+int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts, orctm_nh_t** handle) {
+  if (!handle) return ORCTM_NH_ERR_INVALID;
+  *handle = nullptr;
+  if (!grid_valid(grid)) return ORCTM_NH_ERR_INVALID;
+  orctm_nh_options_t o;
+  orctm_nh_default_options(&o);
+  if (opts) o = *opts;
+  if (o.precision != ORCTM_NH_FP64 && o.precision != ORCTM_NH_MIXED) return ORCTM_NH_ERR_INVALID;
+  if (o.rtol_inner < 0) o.rtol_inner = (o.precision == ORCTM_NH_MIXED) ? 1e-5 : 0.0;
+  if (o.maxit <= 0) o.maxit = 10000;
+  if (o.max_outer <= 0) o.max_outer = 40;
+  if (o.check_every <= 0) o.check_every = 16;
+  if (!(o.rtol > 0)) o.rtol = 1e-10;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device || o.device < 0) return ORCTM_NH_ERR_NODEVICE;
+  if (hipSetDevice(o.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, o.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ORCTM_NH_ERR_NODEVICE;
+  orctm_nh_t* h = new (std::nothrow) orctm_nh_t();
+  if (!h) return ORCTM_NH_ERR_NOMEM;
+  h->opts = o;
+  h->IE = grid->ie;
+  h->JE = grid->je;
+  h->KE = grid->ke;
+  h->nx = grid->ie - 2;
+  h->ny = grid->je - 2;
+  h->nz = grid->ke;
+  h->pitch = (h->nx + 31) / 32 * 32;
+  h->plane = (long long)h->pitch * h->ny;
+  h->fp32 = (o.precision == ORCTM_NH_MIXED);
+  int rc = create_device(h, grid);
+  if (rc != ORCTM_NH_OK) {
+    ctx_destroy(h->ctx);
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *handle = h;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_destroy(orctm_nh_t* h) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(h->opts.device);
+  if (h->ctx.stream) hipStreamSynchronize(h->ctx.stream);
+  ctx_destroy(h->ctx);
+  free_all(h);
+  delete h;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg) {
+  if (!h || !divg) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t n = (size_t)h->IE * h->JE * h->KE;
+  hipError_t e = hipMemcpyAsync(h->d_fort, divg, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  if (e == hipSuccess)
+    e = launch_rhs(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  return map_err(e);
+}
+
+int orctm_nh_set_guess(orctm_nh_t* h, const double* pnh) {
+  if (!h || !pnh) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t n = (size_t)h->IE * h->JE * h->KE;
+  hipError_t e = hipMemcpyAsync(h->d_fort, pnh, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  if (e == hipSuccess)
+    e = launch_guess(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  if (e == hipSuccess) h->guess_set = true;
+  return map_err(e);
+}
+
+int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  const double t0 = now_ms();
+  hipSetDevice(h->opts.device);
+  int rc = prepare_guess(h);
+  if (rc != ORCTM_NH_OK) return rc;
+  float ms = 0.f;
+  hipError_t e = ctx_solve(h->ctx, &ms);
+  if (e != hipSuccess) return map_err(e);
+  fill_stats(h->ctx.h_outer[0], st, ms, now_ms() - t0);
+  return outer_status(h->ctx.h_outer[0]);
+}
+
+int orctm_nh_get_solution(orctm_nh_t* h, double* pnh) {
+  if (!h || !pnh) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  // interior only (mo_petsc.f90:223-227); the halo of pnh is left as it is
+  for (int k = 0; k < h->nz; ++k) {
+    double* dst = pnh + 1 + (size_t)h->IE * (1 + (size_t)h->JE * k);
+    const double* src = h->sys.p64 + (size_t)k * h->plane;
+    hipError_t e = hipMemcpy2D(dst, (size_t)h->IE * 8, src, (size_t)h->pitch * 8, (size_t)h->nx * 8, (size_t)h->ny,
+                               hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return map_err(e);
+  }
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_solve(orctm_nh_t* h, const double* divg, double* pnh, orctm_nh_stats_t* st) {
+  if (!h || !divg || !pnh) return ORCTM_NH_ERR_INVALID;
+  const double t0 = now_ms();
+  int rc = orctm_nh_set_rhs(h, divg);
+  if (rc != ORCTM_NH_OK) return rc;
+  orctm_nh_stats_t s;
+  const int status = orctm_nh_solve_resident(h, &s);
+  if (status == ORCTM_NH_ERR_HIP || status == ORCTM_NH_ERR_NOMEM || status == ORCTM_NH_ERR_INVALID) return status;
+  if (status != ORCTM_NH_ERR_NONFINITE) {
+    rc = orctm_nh_get_solution(h, pnh);
+    if (rc != ORCTM_NH_OK) return rc;
+  }
+  s.total_ms = now_ms() - t0;
+  if (st) *st = s;
+  return status;
+}
+
+int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
+  if (!h || !x || !y) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t bytes = (size_t)h->plane * h->nz * sizeof(double);
+  double *dx = nullptr, *dy = nullptr;
+  hipError_t e = hipMalloc(&dx, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dy, bytes);
+  if (e == hipSuccess) e = hipMemset(dx, 0, bytes);
+  if (e == hipSuccess) e = to_device_vec(h, dx, x, sizeof(double));
+  if (e == hipSuccess) e = launch_apply64(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dx, dy);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  if (e == hipSuccess) e = to_host_vec(h, y, dy, sizeof(double));
+  hipFree(dx);
+  hipFree(dy);
+  return map_err(e);
+}
+
+int orctm_nh_get_operator(orctm_nh_t* h, double* coef7) {
+  if (!h || !coef7) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t NP = (size_t)h->plane * h->nz, N = (size_t)h->nx * h->ny * h->nz;
+  for (int q = 0; q < 7; ++q) {
+    hipError_t e = to_host_vec(h, coef7 + q * N, h->sys.A64 + q * NP, sizeof(double));
+    if (e != hipSuccess) return map_err(e);
+  }
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  if (nx) *nx = h->nx;
+  if (ny) *ny = h->ny;
+  if (nz) *nz = h->nz;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes) {
+  if (!h || !bytes) return ORCTM_NH_ERR_INVALID;
+  *bytes = (int64_t)h->bytes;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
+  if (!h || !x || !y) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t es = h->fp32 ? 4 : 8;
+  Ctx& c = h->ctx;
+  hipError_t e = hipMemset(h->sys.z, 0, (size_t)h->plane * h->nz * es);
+  if (e == hipSuccess) e = to_device_vec(h, h->sys.z, x, es);
+  if (e == hipSuccess) {
+    if (h->fp32) {
+      Inner<float> in{};
+      in.first = 1;
+      in.done = RUN;
+      in.rho = 1.f;
+      e = hipMemcpy(c.d_inner, &in, sizeof(in), hipMemcpyHostToDevice);
+    } else {
+      Inner<double> in{};
+      in.first = 1;
+      in.done = RUN;
+      in.rho = 1.0;
+      e = hipMemcpy(c.d_inner, &in, sizeof(in), hipMemcpyHostToDevice);
+    }
+  }
+  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess) e = to_host_vec(h, y, h->sys.q, es);
+  return map_err(e);
+}
+
+int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
+  if (!h || !r || !z) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  const size_t es = h->fp32 ? 4 : 8;
+  Ctx& c = h->ctx;
+  Outer o{};
+  o.odone = RUN;
+  hipError_t e = hipMemcpy(c.d_outer, &o, sizeof(o), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = to_device_vec(h, h->sys.r, r, es);
+  if (e == hipSuccess) e = launch_init(c.stream, h->fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess) e = to_host_vec(h, z, h->sys.z, es);
+  if (e == hipSuccess && rho) {
+    if (h->fp32) {
+      Inner<float> in;
+      e = hipMemcpy(&in, c.d_inner, sizeof(in), hipMemcpyDeviceToHost);
+      *rho = in.rho;
+    } else {
+      Inner<double> in;
+      e = hipMemcpy(&in, c.d_inner, sizeof(in), hipMemcpyDeviceToHost);
+      *rho = in.rho;
+    }
+  }
+  return map_err(e);
+}
+
+int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_t** batch) {
+  if (!batch || n <= 0 || !systems) return ORCTM_NH_ERR_INVALID;
+  *batch = nullptr;
+  const bool fp32 = systems[0] ? systems[0]->fp32 : true;
+  int max_outer = 0, check = 16, dev = 0;
+  bool prof = false;
+  std::vector<Sys> ss;
+  std::vector<std::vector<Tile>> tl;
+  for (int q = 0; q < n; ++q) {
+    orctm_nh_t* h = systems[q];
+    if (!h || h->fp32 != fp32 || h->opts.device != systems[0]->opts.device) return ORCTM_NH_ERR_INVALID;
+    for (int r = 0; r < q; ++r)
+      if (systems[r] == h) return ORCTM_NH_ERR_INVALID;
+    ss.push_back(h->sys);
+    tl.push_back(h->tiles);
+    max_outer = std::max(max_outer, h->opts.max_outer);
+    check = systems[0]->opts.check_every;
+    dev = h->opts.device;
+    prof |= h->opts.profile != 0;
+  }
+  orctm_nh_batch_t* b = new (std::nothrow) orctm_nh_batch_t();
+  if (!b) return ORCTM_NH_ERR_NOMEM;
+  b->members.assign(systems, systems + n);
+  hipSetDevice(dev);
+  hipError_t e = ctx_build(b->ctx, ss, tl, fp32, check, max_outer, dev, prof);
+  if (e != hipSuccess) {
+    ctx_destroy(b->ctx);
+    delete b;
+    return map_err(e);
+  }
+  *batch = b;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
+  if (!b) return ORCTM_NH_ERR_INVALID;
+  const double t0 = now_ms();
+  hipSetDevice(b->ctx.device);
+  for (orctm_nh_t* h : b->members) {
+    if (!h->opts.warm_start && !h->guess_set) {
+      hipError_t e = hipMemsetAsync(h->sys.p64, 0, (size_t)h->plane * h->nz * sizeof(double), b->ctx.stream);
+      if (e != hipSuccess) return map_err(e);
+    }
+    h->guess_set = false;
+  }
+  float ms = 0.f;
+  hipError_t e = ctx_solve(b->ctx, &ms);
+  if (e != hipSuccess) return map_err(e);
+  const double tot = now_ms() - t0;
+  int rc = ORCTM_NH_OK;
+  for (int q = 0; q < b->ctx.nsys; ++q) {
+    const int s = outer_status(b->ctx.h_outer[q]);
+    if (rc == ORCTM_NH_OK && s != ORCTM_NH_OK) rc = s;
+    if (st) fill_stats(b->ctx.h_outer[q], st + q, ms, tot);
+  }
+  return rc;
+}
+
+static int profile_read(Ctx& c, orctm_nh_profile_t* p, int32_t reset) {
+  if (!p) return ORCTM_NH_ERR_INVALID;
+  memset(p, 0, sizeof(*p));
+  p->matvec_ms = c.mv_ms;
+  p->update_ms = c.up_ms;
+  p->matvec_launches = c.mv_n;
+  p->update_launches = c.up_n;
+  p->cells = c.cells;
+  if (reset) {
+    c.mv_ms = c.up_ms = 0.0;
+    c.mv_n = c.up_n = 0;
+  }
+  return c.prof ? ORCTM_NH_OK : ORCTM_NH_ERR_INVALID;
+}
+
+int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* p, int32_t reset) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return profile_read(h->ctx, p, reset);
+}
+
+int orctm_nh_batch_profile_read(orctm_nh_batch_t* b, orctm_nh_profile_t* p, int32_t reset) {
+  if (!b) return ORCTM_NH_ERR_INVALID;
+  return profile_read(b->ctx, p, reset);
+}
+
+int orctm_nh_batch_destroy(orctm_nh_batch_t* b) {
+  if (!b) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(b->ctx.device);
+  if (b->ctx.stream) hipStreamSynchronize(b->ctx.stream);
+  ctx_destroy(b->ctx);
+  delete b;
+  return ORCTM_NH_OK;
+}
+
+}  // extern "C"

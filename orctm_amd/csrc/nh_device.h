@@ -1,0 +1,79 @@
+// nh_device.h -- device-side data structures of the gfx950 Poisson solver.
+//
+// Layout in HBM (one system): the interior nx x ny x nz cells of the reference DMDA
+// (mo_petsc.f90:48-50,75-95) stored i-fastest with a row pitch padded to a multiple of 32
+// elements: cell (i,j,k) at k*plane + j*pitch + i, plane = pitch*ny. Coefficients are 7 such
+// planes-of-volumes (centre, W, E, N, S, U, D), each nz*plane long. Padding cells are never read
+// as data: every neighbour access goes through the wrap/validity map of the domain.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace onh {
+
+constexpr int TX = 64;           // tile width in i: one wavefront, consecutive columns
+constexpr int TY = 4;            // tile height in j: four wavefronts
+constexpr int NT = TX * TY;      // threads per block
+constexpr int NCOEF = 7;
+
+enum Done : int { RUN = 0, CONVERGED = 1, NOTCONV = 2, BREAKDOWN = 3, NONFINITE = 4, SKIP = 5 };
+
+// FP64 outer-refinement state of one system.
+struct Outer {
+  double nb;          // ||b||_2
+  double nr;          // ||b - A p||_2 of the last residual
+  double nr_prev;     // normaliser of the current inner right-hand side
+  double smax;        // max S over the system (S = DLXP*DLYP/ZP)
+  double rtol, rtol_inner;
+  double inv_gamma;   // scale of the pending inner solution (p += x * inv_gamma)
+  double rel;         // nr / nb
+  int outer, max_outer, inner_total, maxit;
+  int odone;          // Done
+  int pad[3];
+};
+
+// Inner PCG state of one system (scalars in the inner precision T).
+template <typename T>
+struct Inner {
+  T rho, pq, rr, rr0, alpha, beta, tol2, padT;
+  int iter, maxit, done, first, pbuf, pad[3];
+};
+
+// One system as the kernels see it (a copy lives in device memory).
+struct Sys {
+  int nx, ny, nz, pitch;
+  long long plane;
+  int cyc_x, cyc_y;
+  int tiles_x, tiles_y, ntiles, pad0;
+  const double* A64;    // reference operator, 7 planes
+  const double* S;      // symmetrising weight DLXP*DLYP/ZP
+  const void* C;        // inner operator S*(-A) in T, 7 planes
+  const void* invD;     // line preconditioner: 1/D_k
+  const void* L;        // line preconditioner: l_k
+  const uint8_t* wet;   // WETO as 0/1
+  double* b;            // right-hand side DIVG*WETO
+  double* p64;          // solution (FP64)
+  void* x;              // inner solution
+  void* r;              // inner residual (also the inner right-hand side at init)
+  void* z;              // preconditioned residual
+  void* pa;             // search direction, buffer A
+  void* pb;             // search direction, buffer B
+  void* q;              // C p
+  double* partials;     // 2*ntiles slots of block partial sums
+  unsigned* counter;    // last-block counter
+  Outer* outer;
+  void* inner;          // Inner<T>*
+  double* nbp;          // ||b||_2 of the resident right-hand side (written by the rhs kernel)
+  const double* smaxp;  // max S of the system (written by the assembly)
+  double rtol, rtol_inner;
+  int maxit, max_outer;
+};
+
+struct Tile {
+  int sys;    // index into the Sys array of the launch
+  int bx, by; // tile coordinates within the system
+  int local;  // tile index within its system (partials slot)
+};
+
+}  // namespace onh

@@ -1,0 +1,231 @@
+"""Host-side mirror of the reference's MO_PETSC interface, over the C ABI of liborctm_nh.so.
+
+Reference (ORCTM/src_orctm/mo_petsc.f90):
+
+* ``petsc_start``   (:41-128)  -> :func:`petsc_start` / :class:`NHSolver` construction
+* ``petsc_solve_3D`` (:211-233) -> :func:`petsc_solve_3D` / :meth:`NHSolver.solve`
+* ``petsc_end``     (:131-143) -> :func:`petsc_end` / :meth:`NHSolver.close`
+
+Arrays are the caller's Fortran arrays (numpy, order='F', with the halo ring). ``solve`` reads
+``DIVG`` and writes the interior of ``PNH`` only, and warm-starts from the previous solution as
+``KSPSetInitialGuessNonzero`` does (:232). Unlike the reference (which ignores PETSc's ierr), every
+call reports a status; ``solve`` raises :class:`NHError` on errors and returns the statistics.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+
+class NHError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        extra = ""
+        if code in (abi.ERR_HIP, abi.ERR_NOMEM, abi.ERR_NODEVICE):
+            extra = f" [{abi.last_error()}]"
+        super().__init__(f"{what}: {abi.status_name(code)} ({code}){extra}")
+        self.code = code
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optional[float] = None,
+            maxit: int = 10000, max_outer: int = 40, warm_start: bool = True, check_every: int = 16,
+            device: int = 0, profile: bool = False) -> abi.OptionsC:
+    lib = abi.load()
+    o = abi.OptionsC()
+    lib.orctm_nh_default_options(ctypes.byref(o))
+    o.precision = precision
+    o.rtol = rtol
+    o.rtol_inner = -1.0 if rtol_inner is None else rtol_inner
+    o.maxit = maxit
+    o.max_outer = max_outer
+    o.warm_start = 1 if warm_start else 0
+    o.check_every = check_every
+    o.device = device
+    o.profile = 1 if profile else 0
+    return o
+
+
+class NHSolver:
+    """One system on one GPU: petsc_start on construction, petsc_solve_3D per time step."""
+
+    def __init__(self, grid, opts: Optional[abi.OptionsC] = None, **kw):
+        self.lib = abi.load()
+        self.grid = grid
+        self.opts = opts if opts is not None else options(**kw)
+        h = ctypes.c_void_p()
+        rc = self.lib.orctm_nh_create(ctypes.byref(grid.c_struct()), ctypes.byref(self.opts), ctypes.byref(h))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_create")
+        self.h = h
+        self.shape = (grid.IE, grid.JE, grid.KE)
+        self.fp32 = self.opts.precision == abi.MIXED
+
+    # -- the reference's three calls -------------------------------------------------------------
+    def solve(self, divg: np.ndarray, pnh: np.ndarray, check: bool = True) -> dict:
+        """petsc_solve_3D: b = DIVG*WETO, solve, write the PNH interior. Returns the statistics."""
+        self._check_f(divg, "divg")
+        self._check_f(pnh, "pnh")
+        st = abi.StatsC()
+        rc = self.lib.orctm_nh_solve(self.h, _dp(divg), _dp(pnh), ctypes.byref(st))
+        if check and rc not in (abi.OK,):
+            raise NHError(rc, "orctm_nh_solve")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def close(self) -> None:
+        """petsc_end."""
+        if getattr(self, "h", None):
+            self.lib.orctm_nh_destroy(self.h)
+            self.h = None
+
+    # -- resident (split) form -------------------------------------------------------------------
+    def set_rhs(self, divg: np.ndarray) -> None:
+        self._check_f(divg, "divg")
+        self._rc(self.lib.orctm_nh_set_rhs(self.h, _dp(divg)), "orctm_nh_set_rhs")
+
+    def set_guess(self, pnh: np.ndarray) -> None:
+        self._check_f(pnh, "pnh")
+        self._rc(self.lib.orctm_nh_set_guess(self.h, _dp(pnh)), "orctm_nh_set_guess")
+
+    def solve_resident(self) -> dict:
+        st = abi.StatsC()
+        rc = self.lib.orctm_nh_solve_resident(self.h, ctypes.byref(st))
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def get_solution(self, pnh: np.ndarray) -> np.ndarray:
+        self._check_f(pnh, "pnh")
+        self._rc(self.lib.orctm_nh_get_solution(self.h, _dp(pnh)), "orctm_nh_get_solution")
+        return pnh
+
+    # -- verification entry points ---------------------------------------------------------------
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._rc(self.lib.orctm_nh_apply(self.h, _dp(x), _dp(y)), "orctm_nh_apply")
+        return y
+
+    def operator(self) -> np.ndarray:
+        N = self.grid.cells
+        A = np.empty((7, N))
+        self._rc(self.lib.orctm_nh_get_operator(self.h, _dp(A)), "orctm_nh_get_operator")
+        return A
+
+    def inner_apply(self, x: np.ndarray) -> np.ndarray:
+        dt = np.float32 if self.fp32 else np.float64
+        x = np.ascontiguousarray(x, dtype=dt)
+        y = np.empty_like(x)
+        self._rc(self.lib.orctm_nh_inner_apply(self.h, x.ctypes.data, y.ctypes.data), "orctm_nh_inner_apply")
+        return y
+
+    def inner_precond(self, r: np.ndarray):
+        dt = np.float32 if self.fp32 else np.float64
+        r = np.ascontiguousarray(r, dtype=dt)
+        z = np.empty_like(r)
+        rho = ctypes.c_double()
+        self._rc(self.lib.orctm_nh_inner_precond(self.h, r.ctypes.data, z.ctypes.data, ctypes.byref(rho)),
+                 "orctm_nh_inner_precond")
+        return z, rho.value
+
+    def profile(self, reset: bool = True) -> dict:
+        p = abi.ProfileC()
+        self._rc(self.lib.orctm_nh_profile_read(self.h, ctypes.byref(p), 1 if reset else 0), "orctm_nh_profile_read")
+        return {"matvec_ms": p.matvec_ms, "update_ms": p.update_ms, "matvec_launches": p.matvec_launches,
+                "update_launches": p.update_launches, "cells": p.cells}
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64()
+        self._rc(self.lib.orctm_nh_device_bytes(self.h, ctypes.byref(b)), "orctm_nh_device_bytes")
+        return b.value
+
+    # -- helpers ---------------------------------------------------------------------------------
+    def _check_f(self, a: np.ndarray, name: str) -> None:
+        if a.dtype != np.float64 or a.shape != self.shape or not a.flags["F_CONTIGUOUS"]:
+            raise ValueError(f"{name} must be a Fortran-ordered float64 array of shape {self.shape}")
+
+    @staticmethod
+    def _rc(rc: int, what: str) -> None:
+        if rc != abi.OK:
+            raise NHError(rc, what)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NHBatch:
+    """Independent systems (ensemble members, nests) advanced together, one launch per phase."""
+
+    def __init__(self, solvers: Sequence[NHSolver]):
+        self.lib = abi.load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        b = ctypes.c_void_p()
+        rc = self.lib.orctm_nh_batch_create(len(self.solvers), arr, ctypes.byref(b))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_batch_create")
+        self.b = b
+
+    def solve(self):
+        st = (abi.StatsC * len(self.solvers))()
+        rc = self.lib.orctm_nh_batch_solve(self.b, st)
+        return rc, [s.as_dict() for s in st]
+
+    def profile(self, reset: bool = True) -> dict:
+        p = abi.ProfileC()
+        rc = self.lib.orctm_nh_batch_profile_read(self.b, ctypes.byref(p), 1 if reset else 0)
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_batch_profile_read")
+        return {"matvec_ms": p.matvec_ms, "update_ms": p.update_ms, "matvec_launches": p.matvec_launches,
+                "update_launches": p.update_launches, "cells": p.cells}
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.orctm_nh_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# Module-level mirror of MO_PETSC for callers written against the reference's subroutines.
+_state = {}
+
+
+def petsc_start(grid, **kw) -> NHSolver:
+    """mo_petsc.f90:41-128: create the solver and assemble the operator once."""
+    s = NHSolver(grid, **kw)
+    _state["solver"] = s
+    return s
+
+
+def petsc_solve_3D(divg: np.ndarray, pnh: np.ndarray) -> dict:
+    """mo_petsc.f90:211-233: solve for PNH from DIVG, warm-started from the previous solution."""
+    return _state["solver"].solve(divg, pnh)
+
+
+def petsc_end() -> None:
+    """mo_petsc.f90:131-143."""
+    s = _state.pop("solver", None)
+    if s is not None:
+        s.close()

@@ -1,0 +1,228 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the same seeded inputs.
+
+Bars: bit-exact for the assembled reference operator, the FP64 matvec, the inner matvec and the line
+preconditioner (same floating-point operations in the same order); for full solves, the true relative
+residual ||b - A p|| / ||b|| computed by the oracle's FP64 operator must be <= 1e-10, and the solution
+must match an FP64 direct solve (SciPy SuperLU) to ||p - p_direct|| / ||p_direct|| <= 1e-7.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle
+from cases import SPECS, build
+from orctm_amd import abi, synth
+from orctm_amd.nh import NHBatch, NHSolver, options
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+SOL_TOL = 1e-7
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.dtype != b.dtype or a.shape != b.shape:
+        return False
+    v = np.uint64 if a.dtype == np.float64 else np.uint32
+    return np.array_equal(a.view(v), b.view(v))
+
+
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        g, divg = build(name)
+        A, S = oracle.assemble(g)
+        b = oracle.rhs(g, divg)
+        _cache[name] = (g, divg, A, S, b)
+    return _cache[name]
+
+
+def direct(name):
+    key = name + "/direct"
+    if key not in _cache:
+        g, divg, A, S, b = case(name)
+        _cache[key] = spla.spsolve(oracle.to_scipy(g, A).tocsc(), b)
+    return _cache[key]
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_operator_bit_exact(name, precision):
+    g, divg, A, S, b = case(name)
+    print(f"[operator] {name} precision={precision}", flush=True)
+    with NHSolver(g, options(precision=precision)) as s:
+        assert bits_equal(s.operator(), A)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_apply64_bit_exact(name):
+    g, divg, A, S, b = case(name)
+    print(f"[apply64] {name}", flush=True)
+    x = synth.normal(99, 1, g.cells)
+    with NHSolver(g, options(precision=abi.FP64)) as s:
+        assert bits_equal(s.apply(x), oracle.matvec(g, A, x))
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_inner_operator_and_line_preconditioner_bit_exact(name, precision):
+    g, divg, A, S, b = case(name)
+    print(f"[inner] {name} precision={precision}", flush=True)
+    dt = np.float32 if precision == abi.MIXED else np.float64
+    C = oracle.inner_coef(A, S, dt)
+    invD, L = oracle.line_factor(g, C)
+    x = synth.normal(98, 2, g.cells).astype(dt)
+    with NHSolver(g, options(precision=precision)) as s:
+        assert bits_equal(s.inner_apply(x), oracle.matvec(g, C, x))
+        z, rho = s.inner_precond(x)
+        z_ref, rho_ref = oracle.line_apply(g, invD, L, x)
+        assert bits_equal(z, z_ref)
+        tol = 1e-5 if precision == abi.MIXED else 1e-12
+        assert abs(rho - rho_ref) <= tol * abs(rho_ref)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_solve_matches_direct_solve(name, precision):
+    g, divg, A, S, b = case(name)
+    print(f"[solve] {name} precision={precision}", flush=True)
+    pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+    pnh[0, :, :] = 7.0  # halo must be left untouched (mo_petsc.f90:223-227 writes the interior only)
+    with NHSolver(g, options(precision=precision)) as s:
+        st = s.solve(divg, pnh)
+    assert st["status"] == "OK", st
+    assert np.all(pnh[0, :, :] == 7.0)
+    p = g.interior(pnh)
+    rel = oracle.rel_res(g, A, p, b)
+    assert rel <= RTOL, (rel, st)
+    assert abs(rel - st["rel_res"]) <= 1e-3 * RTOL + 1e-3 * rel
+    pd = direct(name)
+    err = np.linalg.norm(p - pd) / np.linalg.norm(pd)
+    assert err <= SOL_TOL, err
+
+
+def test_mixed_and_fp64_agree_and_oracle_iteration_counts():
+    g, divg, A, S, b = case("regional")
+    out = {}
+    for prec in (abi.FP64, abi.MIXED):
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=prec)) as s:
+            st = s.solve(divg, pnh)
+        out[prec] = (g.interior(pnh), st)
+        p_cpu, st_cpu = oracle.solve(g, A, S, b, precision=prec)
+        # same algorithm: iteration counts agree up to reduction-order effects
+        assert abs(st["inner_iters"] - st_cpu.inner_iters) <= max(3, st_cpu.inner_iters // 10)
+        assert st["outer_iters"] == st_cpu.outer_iters or abs(st["outer_iters"] - st_cpu.outer_iters) <= 1
+    p64, pmx = out[abi.FP64][0], out[abi.MIXED][0]
+    assert np.linalg.norm(pmx - p64) / np.linalg.norm(p64) <= SOL_TOL
+
+
+def test_warm_start_resolve_is_idempotent():
+    g, divg, A, S, b = case("regional")
+    pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+    with NHSolver(g, options(precision=abi.MIXED, warm_start=True)) as s:
+        st1 = s.solve(divg, pnh)
+        p1 = g.interior(pnh).copy()
+        st2 = s.solve(divg, pnh)  # warm start from the converged solution: nothing left to do
+    assert st1["status"] == "OK" and st2["status"] == "OK"
+    assert st2["inner_iters"] == 0 and st2["outer_iters"] == 0
+    assert bits_equal(g.interior(pnh), p1)
+
+
+def test_set_guess_and_cold_start_options():
+    g, divg, A, S, b = case("cyclic_x")
+    pd = direct("cyclic_x")
+    guess = g.to_fortran(pd)
+    with NHSolver(g, options(precision=abi.FP64, warm_start=False)) as s:
+        s.set_rhs(divg)
+        s.set_guess(guess)
+        st = s.solve_resident()
+        assert st["status"] == "OK" and st["inner_iters"] == 0
+        st = s.solve_resident()  # warm_start off: starts from zero again
+        assert st["status"] == "OK" and st["inner_iters"] > 0
+
+
+def test_zero_rhs_gives_zero_solution():
+    g, divg, A, S, b = case("closed")
+    pnh = np.full((g.IE, g.JE, g.KE), 3.0, order="F")
+    with NHSolver(g, options(precision=abi.MIXED)) as s:
+        st = s.solve(np.zeros_like(divg), pnh)
+    assert st["status"] == "OK" and st["rel_res"] == 0.0
+    assert np.all(g.interior(pnh) == 0.0)
+
+
+def test_all_land_system():
+    spec = synth.GridSpec(16, 8, 5, bathy="flat", depth=0.0, seed=3)
+    g, divg = synth.build(spec)
+    assert g.weto.sum() == 0
+    pnh = np.ones((g.IE, g.JE, g.KE), order="F")
+    with NHSolver(g, options(precision=abi.MIXED)) as s:
+        st = s.solve(divg, pnh)
+    assert st["status"] == "OK"
+    assert np.all(g.interior(pnh) == 0.0)
+
+
+def test_nonfinite_rhs_is_reported_and_pnh_untouched():
+    g, divg, A, S, b = case("closed")
+    bad = divg.copy(order="F")
+    wet = np.argwhere(g.weto[1:-1, 1:-1, :] == 1.0)[0] + np.array([1, 1, 0])
+    bad[tuple(wet)] = np.nan
+    pnh = np.full((g.IE, g.JE, g.KE), 5.0, order="F")
+    with NHSolver(g, options(precision=abi.MIXED)) as s:
+        st = s.solve(bad, pnh, check=False)
+    assert st["rc"] == abi.ERR_NONFINITE
+    assert np.all(pnh == 5.0)
+
+
+def test_maxit_exhaustion_returns_status():
+    g, divg, A, S, b = case("transect")
+    pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+    with NHSolver(g, options(precision=abi.FP64, maxit=5)) as s:
+        st = s.solve(divg, pnh, check=False)
+    assert st["rc"] == abi.NOT_CONVERGED
+    assert st["inner_iters"] <= 5 + 16
+    assert np.all(np.isfinite(pnh))
+
+
+def test_invalid_grid_rejected():
+    g, divg = synth.build(synth.GridSpec(12, 6, 4, seed=4))
+    g.dlxp[3, 3] = 0.0
+    with pytest.raises(Exception) as ei:
+        NHSolver(g, options(precision=abi.MIXED))
+    assert getattr(ei.value, "code", None) == abi.ERR_INVALID
+
+
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+def test_batch_results_identical_alone_and_in_the_mix(precision):
+    names = ["regional", "transect", "cyclic_x", "closed", "stretched_dx"]
+    alone = {}
+    for n in names:
+        g, divg, A, S, b = case(n)
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=precision)) as s:
+            st = s.solve(divg, pnh)
+        assert st["status"] == "OK"
+        alone[n] = (g.interior(pnh), st)
+    solvers = []
+    for n in names:
+        g, divg, A, S, b = case(n)
+        s = NHSolver(g, options(precision=precision))
+        s.set_rhs(divg)
+        solvers.append(s)
+    batch = NHBatch(solvers)
+    rc, stats = batch.solve()
+    assert rc == abi.OK, stats
+    for n, s, st in zip(names, solvers, stats):
+        g = s.grid
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        s.get_solution(pnh)
+        assert bits_equal(g.interior(pnh), alone[n][0]), n
+        assert st["inner_iters"] == alone[n][1]["inner_iters"]
+        assert st["outer_iters"] == alone[n][1]["outer_iters"]
+    batch.close()
+    for s in solvers:
+        s.close()

@@ -1,0 +1,107 @@
+"""Pins the oracle's solver pieces (the algorithm the HIP library runs) on the CPU:
+
+* the vertical-line preconditioner against scipy.linalg.solve_banded on the tridiagonal of C = S(-A);
+* r.M^{-1}r from the LDL^T forward pass against the explicit dot product;
+* the FP64 and mixed refinement solves and FGMRES(30) against SciPy SuperLU direct solves;
+* fault handling: NaN right-hand side, zero right-hand side, maxit exhaustion.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+import oracle
+from cases import SPECS, build
+from orctm_amd import synth
+
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        g, divg = build(name)
+        A, S = oracle.assemble(g)
+        b = oracle.rhs(g, divg)
+        _cache[name] = (g, divg, A, S, b)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", ["regional", "transect", "stretched_dx", "one_layer"])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_line_preconditioner_matches_banded_solve(name, dt):
+    g, divg, A, S, b = case(name)
+    Cm = oracle.inner_coef(A, S, dt)
+    invD, L = oracle.line_factor(g, Cm)
+    r = synth.normal(5, 9, g.cells).astype(dt)
+    z, rho = oracle.line_apply(g, invD, L, r)
+    P = g.nx * g.ny
+    nz = g.nz
+    C64 = Cm.astype(np.float64)
+    tol = 1e-12 if dt == np.float64 else 2e-4
+    for col in list(range(0, P, max(1, P // 25))) + [P - 1]:
+        idx = col + P * np.arange(nz)
+        d = C64[0, idx]
+        # symmetric off-diagonal between k-1 and k: C_up(k) where both C_down(k-1) and C_up(k) are non-zero
+        e = np.where((C64[6, idx[:-1]] != 0) & (C64[5, idx[1:]] != 0), C64[5, idx[1:]], 0.0)
+        ab = np.zeros((3, nz))
+        ab[0, 1:] = e
+        ab[1, :] = d
+        ab[2, :-1] = e
+        zr = sla.solve_banded((1, 1), ab, r[idx].astype(np.float64))
+        assert np.allclose(z[idx], zr, rtol=tol, atol=tol * np.abs(zr).max())
+    rho_ref = float(np.dot(r.astype(np.float64), z.astype(np.float64)))
+    assert abs(rho - rho_ref) <= (1e-12 if dt == np.float64 else 1e-4) * abs(rho_ref)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_refinement_solve_matches_direct(name, precision):
+    g, divg, A, S, b = case(name)
+    p, st = oracle.solve(g, A, S, b, precision=precision)
+    assert st.status == 0, st.as_dict()
+    rel = oracle.rel_res(g, A, p, b)
+    assert rel <= 1e-10
+    assert abs(rel - st.rel_res) <= 1e-12
+    pd = spla.spsolve(oracle.to_scipy(g, A).tocsc(), b)
+    assert np.linalg.norm(p - pd) / np.linalg.norm(pd) <= 1e-7
+    if precision == 1:
+        assert st.outer_iters >= 2  # FP32 inner alone cannot reach 1e-10
+
+
+@pytest.mark.parametrize("name", ["regional", "transect", "closed"])
+def test_fgmres_matches_direct(name):
+    g, divg, A, S, b = case(name)
+    x, st = oracle.fgmres(g, A, S, b, restart=30, rtol=1e-10)
+    assert st.status == 0
+    pd = spla.spsolve(oracle.to_scipy(g, A).tocsc(), b)
+    assert oracle.rel_res(g, A, x, b) <= 1e-10
+    assert np.linalg.norm(x - pd) / np.linalg.norm(pd) <= 1e-7
+
+
+def test_manufactured_solution():
+    g, divg, A, S, b = case("regional")
+    pstar = synth.normal(3, 3, g.cells) * (g.interior(g.weto) == 1.0)
+    bm = oracle.matvec(g, A, pstar)
+    p, st = oracle.solve(g, A, S, bm, precision=1)
+    assert st.status == 0
+    assert np.linalg.norm(p - pstar) / np.linalg.norm(pstar) <= 1e-7
+
+
+def test_faults_nan_zero_maxit():
+    g, divg, A, S, b = case("closed")
+    bad = b.copy()
+    bad[np.flatnonzero(b)[0]] = np.nan
+    p, st = oracle.solve(g, A, S, bad, precision=1)
+    assert st.status == 3  # ERR_NONFINITE
+    p, st = oracle.solve(g, A, S, np.zeros_like(b), precision=1, x0=np.ones(g.cells))
+    assert st.status == 0 and np.all(p == 0.0)
+    g2, divg2, A2, S2, b2 = case("transect")
+    p, st = oracle.solve(g2, A2, S2, b2, precision=0, maxit=5)
+    assert st.status == 1 and st.inner_iters <= 5
+
+
+def test_warm_start_from_solution_takes_no_iterations():
+    g, divg, A, S, b = case("cyclic_x")
+    p, st = oracle.solve(g, A, S, b, precision=1)
+    p2, st2 = oracle.solve(g, A, S, b, precision=1, x0=p)
+    assert st2.inner_iters == 0 and np.array_equal(p, p2)
