@@ -82,7 +82,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="regional", choices=["regional"])
+    ap.add_argument("--workload", default="regional", choices=["regional", "batch", "transect"])
+    ap.add_argument("--nsys", type=int, default=64, help="batch workload: number of systems")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
     args = ap.parse_args()
@@ -91,6 +92,10 @@ def main():
               "across GPUs with no exchange; see the design notes).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "transect":
+        return bench_transect(args)
     spec = synth.config2()
     if args.scale != 1.0:
         spec.nx = max(8, int(spec.nx * args.scale))
@@ -120,16 +125,16 @@ def main():
     inner = statistics.mean(st["inner_iters"] for st in mx["stats"])
     inner64 = statistics.mean(st["inner_iters"] for st in f64["stats"])
     cells = g.cells
-    # roofline of the fused matvec kernel (dominant per-iteration kernel), FP32:
-    # reads 7 coefficients + z + p_old, writes p_new + q = 11 words = 44 B per cell
+    # roofline of the matvec kernel k_bwd (back substitution + p update + 7-point matvec + p.q), FP32:
+    # reads w, l, p_old and 7 coefficients, writes p and q = 12 words = 48 B per cell.
+    # k_fwd (x/r update + forward elimination + r.z, r.r): reads x p r q l invD, writes x r w = 36 B.
     prof = mx["prof"]
     mv_avg_ms = prof["matvec_ms"] / max(prof["matvec_launches"], 1)
     up_avg_ms = prof["update_ms"] / max(prof["update_launches"], 1)
-    mv_bytes = 44.0 * cells
-    up_bytes = 40.0 * cells  # x, p, r, q, l, invD, l(k+1) reads + x, r, z writes (10 words)
-    achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
+    achieved = 48.0 * cells / (mv_avg_ms * 1e-3) / 1e9
     prof64 = f64["prof"]
     mv64_ms = prof64["matvec_ms"] / max(prof64["matvec_launches"], 1)
+    up64_ms = prof64["update_ms"] / max(prof64["update_launches"], 1)
     err = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / max(np.linalg.norm(sols["fp64"]), 1e-300))
 
     line = {
@@ -161,23 +166,155 @@ def main():
         "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "latency_ms_p50_fp64": round(pct(f64["lat"], 50), 3),
         "latency_ms_p99_fp64": round(pct(f64["lat"], 99), 3),
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<float> (fused p-update + 7-point matvec + p.q)",
+        "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (back substitution + p update + 7-point matvec + p.q)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": 44, "avg_launch_ms": round(mv_avg_ms, 5),
+                     "bytes_per_cell": 48, "avg_launch_ms": round(mv_avg_ms, 5),
                      "launches": prof["matvec_launches"]},
         "other_kernels": {
-            "k_column<float> (x/r update + line solve)": {
-                "avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 40,
-                "GBps": round(up_bytes / (up_avg_ms * 1e-3) / 1e9, 1)},
-            "k_spmv<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 88,
-                               "GBps": round(88.0 * cells / (mv64_ms * 1e-3) / 1e9, 1)},
+            "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 36,
+                             "GBps": round(36.0 * cells / (up_avg_ms * 1e-3) / 1e9, 1)},
+            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 96,
+                              "GBps": round(96.0 * cells / (mv64_ms * 1e-3) / 1e9, 1)},
+            "k_fwd<double>": {"avg_launch_ms": round(up64_ms, 5), "bytes_per_cell": 72,
+                              "GBps": round(72.0 * cells / (up64_ms * 1e-3) / 1e9, 1)},
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle FP64 solve on the host cores) ...")
         line["cpu_baseline"] = cpu_baseline(g, divg)
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """64 independent systems of uneven sizes (nx, ny ~ U{96..512}, nz ~ U{30..60}) solved together,
+    one launch per solver phase for the whole batch, per-system convergence masks, hipGraph chunks."""
+    from orctm_amd.nh import NHBatch
+
+    specs = synth.config4_sizes(n=args.nsys)
+    if args.scale != 1.0:
+        for sp in specs:
+            sp.nx = max(8, int(sp.nx * args.scale))
+            sp.ny = max(8, int(sp.ny * args.scale))
+    t = time.perf_counter()
+    systems = []
+    for sp in specs:
+        g, divg = synth.build(sp)
+        systems.append((g, divg))
+    cells = sum(g.cells for g, _ in systems)
+    log(f"[bench] batch: {len(systems)} systems, {cells} cells, generated in {time.perf_counter() - t:.1f} s")
+    out = {}
+    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        solvers = []
+        for g, divg in systems:
+            s = NHSolver(g, options(precision=prec, warm_start=False, profile=True))
+            s.set_rhs(divg)
+            solvers.append(s)
+        batch = NHBatch(solvers)
+        for _ in range(args.warmup):
+            rc, st = batch.solve()
+            if rc != abi.OK:
+                raise RuntimeError(f"batch warmup failed: {rc} {abi.last_error()}")
+        batch.profile(reset=True)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            rc, st = batch.solve()
+            lat.append((time.perf_counter() - a) * 1e3)
+            if rc != abi.OK:
+                raise RuntimeError(f"batch solve failed: {rc}")
+        wall = time.perf_counter() - t0
+        prof = batch.profile(reset=True)
+        inner = sum(x["inner_iters"] for x in st)
+        # cell-iterations actually processed per solve (converged systems drop out of later launches)
+        cell_iters = sum(x["inner_iters"] * g.cells for x, (g, _) in zip(st, systems))
+        # one-by-one reference timing of the same systems (sequential, same kernels)
+        t1 = time.perf_counter()
+        for s in solvers:
+            r = s.solve_resident()
+            if r["rc"] != abi.OK:
+                raise RuntimeError(f"single solve failed: {r}")
+        seq = time.perf_counter() - t1
+        out[name] = dict(wall=wall, lat=lat, inner=inner, cell_iters=cell_iters,
+                         max_inner=max(x["inner_iters"] for x in st),
+                         rel=max(x["rel_res"] for x in st), prof=prof, seq=seq)
+        log(f"[bench] batch {name}: {len(systems) * args.steps / wall:.2f} solves/s, one-by-one "
+            f"{len(systems) / seq:.2f} solves/s, sum inner {inner}, max rel {out[name]['rel']:.2e}")
+        batch.close()
+        for s in solvers:
+            s.close()
+    mx, f64 = out["mixed"], out["fp64"]
+    n = len(systems)
+    prof = mx["prof"]
+    # bytes moved by all matvec launches of the timed solves / their summed device time
+    mv_gbs = 48.0 * mx["cell_iters"] * args.steps / (prof["matvec_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "data": "synthetic (seeded generator, regional-tile recipe per system)",
+        "config": {"workload": f"batch of {n} systems, nx,ny~U{{96..512}}, nz~U{{30..60}}, mixed, rel.res 1e-10",
+                   "cells": cells},
+        "fp64_solves_per_s": round(n * args.steps / f64["wall"], 3),
+        "mixed_over_fp64": round(f64["wall"] / mx["wall"], 3),
+        "one_by_one_solves_per_s_mixed": round(n / mx["seq"], 3),
+        "cell_iters_per_s": round(mx["cell_iters"] * args.steps / mx["wall"], 1),
+        "max_rel_res_mixed": mx["rel"], "max_rel_res_fp64": f64["rel"],
+        "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
+        "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "48 B per active cell-iteration over the summed matvec launch time"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_transect(args):
+    """The ISW transect 512 x 4 (periodic y) x 100: the reference's CPU method (FGMRES(30), here the
+    oracle's restatement with the line preconditioner, FP64) timed on the host beside the GPU solves."""
+    import oracle
+
+    spec = synth.config1()
+    g, divg = synth.build(spec)
+    A, S = oracle.assemble(g)
+    b = oracle.rhs(g, divg)
+    t = time.perf_counter()
+    reps = 0
+    while True:
+        x, st = oracle.fgmres(g, A, S, b, restart=30, rtol=1e-10)
+        reps += 1
+        if time.perf_counter() - t > 5.0 or reps >= 20:
+            break
+    cpu_s = (time.perf_counter() - t) / reps
+    res = {}
+    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        s = NHSolver(g, options(precision=prec, warm_start=False, profile=True))
+        s.set_rhs(divg)
+        wall, lat, stats, prof = time_solves(s, args.steps, args.warmup)
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        s.get_solution(pnh)
+        res[name] = dict(wall=wall, lat=lat, stats=stats,
+                         err=float(np.linalg.norm(g.interior(pnh) - x) / np.linalg.norm(x)),
+                         rel=oracle.rel_res(g, A, g.interior(pnh), b))
+        s.close()
+    mx = res["mixed"]
+    line = {
+        "metric": METRIC, "value": round(args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "data": "synthetic (ISW transect recipe)",
+        "config": {"workload": "ISW transect 512x4x100 periodic y, Gaussian ridge", "cells": g.cells},
+        "fp64_solves_per_s": round(args.steps / res["fp64"]["wall"], 3),
+        "inner_iters_mixed": mx["stats"][-1]["inner_iters"], "inner_iters_fp64": res["fp64"]["stats"][-1]["inner_iters"],
+        "oracle_rel_res_mixed": mx["rel"], "oracle_rel_res_fp64": res["fp64"]["rel"],
+        "rel_err_vs_cpu_fgmres": {k: v["err"] for k, v in res.items()},
+        "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
+        "cpu_baseline": {"value": round(1.0 / cpu_s, 4), "unit": "solves/s", "cores": oracle.num_threads(),
+                         "kind": "port", "sample": f"FGMRES(30) + line preconditioner, FP64, rel.res 1e-10, "
+                                                   f"{st.inner_iters} iterations, {reps} solves"},
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -152,8 +152,19 @@ int orctm_nh_get_operator(orctm_nh_t* h, double* coef7);
 /* Size of the interior: nx = IE-2, ny = JE-2, nz = KE. */
 int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz);
 
-/* Bytes of device memory the handle holds. */
+/* Bytes of device memory the handle holds (its own state plus the operator it uses). */
 int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes);
+
+/* A new handle on the same grid that shares src's assembled operator (read only, freed with the last
+ * handle using it) and owns its own right-hand side (a copy of src's), solution (zero) and work
+ * vectors: about 40 B/cell (MIXED) instead of ~140. For many states on one grid: ensemble members,
+ * serving requests. No counterpart in the reference (one KSP per run, mo_petsc.f90:24-26). */
+int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out);
+
+/* Synthetic time-step forcing on the resident right-hand side: b <- b * (1 + amplitude * xi),
+ * xi ~ N(0,1) per cell from a counter-based stream keyed by (seed, cell); land stays 0. Used by the
+ * serving workload (each request's RHS perturbed by 1 % per step). No counterpart in the reference. */
+int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed);
 
 /* Device time of the two inner-iteration kernels, accumulated over the launches that did work
  * (HIP events recorded on the solver's own stream, inside the captured iteration graph). Needs

@@ -68,7 +68,7 @@ EXPORTS = [
     "orctm_nh_solve_resident", "orctm_nh_get_solution", "orctm_nh_apply", "orctm_nh_get_operator",
     "orctm_nh_get_dims", "orctm_nh_device_bytes", "orctm_nh_inner_apply", "orctm_nh_inner_precond",
     "orctm_nh_batch_create", "orctm_nh_batch_solve", "orctm_nh_batch_destroy", "orctm_nh_profile_read",
-    "orctm_nh_batch_profile_read",
+    "orctm_nh_batch_profile_read", "orctm_nh_clone", "orctm_nh_perturb_rhs",
 ]
 
 _lib = None
@@ -108,6 +108,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.orctm_nh_batch_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     lib.orctm_nh_batch_solve.argtypes = [vp, ctypes.POINTER(StatsC)]
     lib.orctm_nh_batch_destroy.argtypes = [vp]
+    lib.orctm_nh_clone.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.orctm_nh_perturb_rhs.argtypes = [vp, ctypes.c_double, ctypes.c_uint64]
     lib.orctm_nh_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
     lib.orctm_nh_batch_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
     for name in EXPORTS:

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -26,7 +27,6 @@ namespace {
 struct Ctx {
   int nsys = 0;
   bool fp32 = true;
-  int nzr = 0;
   int ntiles = 0;
   int max_outer = 0;
   int maxit = 10000;
@@ -63,8 +63,22 @@ int inner_done(const Ctx& c, int q) {
 
 }  // namespace
 
+// Device arrays of one assembled operator (A64, S, C, line factors, wet mask, max S). Shared, read
+// only, by every handle cloned from the one that assembled it.
+struct OpData {
+  int device = 0;
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  ~OpData() {
+    if (allocs.empty()) return;
+    hipSetDevice(device);
+    for (void* p : allocs) hipFree(p);
+  }
+};
+
 struct orctm_nh {
   orctm_nh_options_t opts;
+  std::shared_ptr<OpData> op;
   int IE = 0, JE = 0, KE = 0, nx = 0, ny = 0, nz = 0, pitch = 0;
   long long plane = 0;
   bool fp32 = true;
@@ -117,7 +131,57 @@ hipError_t dalloc(orctm_nh_t* h, void** p, size_t bytes) {
   if (e != hipSuccess) return e;
   h->allocs.push_back(*p);
   h->bytes += bytes;
-  return hipMemset(*p, 0, bytes);
+  // the solver streams are non-blocking: finish the zero-fill before any of them can touch it
+  e = hipMemset(*p, 0, bytes);
+  return e == hipSuccess ? hipDeviceSynchronize() : e;
+}
+
+hipError_t opalloc(orctm_nh_t* h, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return e;
+  h->op->allocs.push_back(*p);
+  h->op->bytes += bytes;
+  e = hipMemset(*p, 0, bytes);
+  return e == hipSuccess ? hipDeviceSynchronize() : e;
+}
+
+// Per-handle state: right-hand side, solution, inner work vectors, block partials, ||b||.
+int alloc_state(orctm_nh_t* h) {
+  const size_t NP = (size_t)h->plane * h->nz;
+  const size_t es = h->fp32 ? sizeof(float) : sizeof(double);
+  Sys& s = h->sys;
+  void* p;
+  hipError_t e;
+#define SALLOC(field, bytes, type)             \
+  do {                                         \
+    e = dalloc(h, &p, (bytes));                \
+    if (e != hipSuccess) return map_err(e);    \
+    s.field = reinterpret_cast<type>(p);       \
+  } while (0)
+  SALLOC(b, NP * sizeof(double), double*);
+  SALLOC(p64, NP * sizeof(double), double*);
+  SALLOC(x, NP * es, void*);
+  SALLOC(r, NP * es, void*);
+  SALLOC(z, NP * es, void*);
+  SALLOC(pa, NP * es, void*);
+  SALLOC(pb, NP * es, void*);
+  SALLOC(q, NP * es, void*);
+  SALLOC(partials, 2 * (size_t)s.ntiles * sizeof(double), double*);
+  SALLOC(nbp, sizeof(double), double*);
+#undef SALLOC
+  h->d_nb = s.nbp;
+  return ORCTM_NH_OK;
+}
+
+// Staging buffer for a Fortran (IE,JE,KE) array, allocated on first use.
+hipError_t ensure_fort(orctm_nh_t* h) {
+  if (h->d_fort) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = dalloc(h, &p, (size_t)h->IE * h->JE * h->KE * sizeof(double));
+  if (e == hipSuccess) h->d_fort = static_cast<double*>(p);
+  return e;
 }
 
 void free_all(orctm_nh_t* h) {
@@ -175,16 +239,13 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   c.max_outer = max_outer;
   c.device = device;
   for (const Sys& s : systems) c.maxit = std::max(c.maxit, s.maxit);
-  int maxnz = 1;
   std::vector<Tile> nat;
   for (int q = 0; q < c.nsys; ++q) {
-    maxnz = std::max(maxnz, systems[q].nz);
     for (Tile t : tiles[q]) {
       t.sys = q;
       nat.push_back(t);
     }
   }
-  c.nzr = pick_nzr(fp32, maxnz);
   std::vector<Tile> ord = xcd_order(nat);
   c.ntiles = (int)ord.size();
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -218,9 +279,9 @@ hipError_t ctx_chunk(Ctx& c) {
   if (c.prof) {
     for (int it = 0; it < c.check_every; ++it) {
       HIPCHK(hipEventRecord(c.kev[3 * it], c.stream));
-      HIPCHK(launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+      HIPCHK(launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
       HIPCHK(hipEventRecord(c.kev[3 * it + 1], c.stream));
-      HIPCHK(launch_update(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles));
+      HIPCHK(launch_update(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
       HIPCHK(hipEventRecord(c.kev[3 * it + 2], c.stream));
     }
     return hipSuccess;
@@ -230,8 +291,8 @@ hipError_t ctx_chunk(Ctx& c) {
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int it = 0; it < c.check_every && e == hipSuccess; ++it) {
-      e = launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles);
-      if (e == hipSuccess) e = launch_update(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles);
+      e = launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys);
+      if (e == hipSuccess) e = launch_update(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys);
     }
     hipError_t e2 = hipStreamEndCapture(c.stream, &g);
     if (e != hipSuccess) {
@@ -250,14 +311,14 @@ hipError_t ctx_chunk(Ctx& c) {
 hipError_t ctx_solve(Ctx& c, float* ms) {
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   HIPCHK(launch_begin(c.stream, c.fp32, c.d_sys, c.nsys));
-  HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+  HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
   for (int m = 0; m <= c.max_outer + 1; ++m) {
     HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     bool any = false;
     for (int q = 0; q < c.nsys; ++q) any |= (c.h_outer[q].odone == RUN);
     if (!any) break;
-    HIPCHK(launch_init(c.stream, c.fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles));
+    HIPCHK(launch_init(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
     std::fill(c.iter_seen.begin(), c.iter_seen.end(), 0);
     // every inner solve ends within maxit iterations (device-side NOTCONV); the host bound below
     // only guarantees termination of this loop whatever the device flags say
@@ -288,7 +349,7 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
       if (!run) break;
     }
     HIPCHK(launch_refine(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
-    HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
+    HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
   }
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
@@ -350,7 +411,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   hipError_t e;
 #define ALLOC(field, bytes, type)              \
   do {                                         \
-    e = dalloc(h, &p, (bytes));                \
+    e = opalloc(h, &p, (bytes));               \
     if (e != hipSuccess) return map_err(e);    \
     s.field = reinterpret_cast<type>(p);       \
   } while (0)
@@ -360,28 +421,11 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   ALLOC(invD, NP * es, const void*);
   ALLOC(L, NP * es, const void*);
   ALLOC(wet, NP, const uint8_t*);
-  ALLOC(b, NP * sizeof(double), double*);
-  ALLOC(p64, NP * sizeof(double), double*);
-  ALLOC(x, NP * es, void*);
-  ALLOC(r, NP * es, void*);
-  ALLOC(z, NP * es, void*);
-  ALLOC(pa, NP * es, void*);
-  ALLOC(pb, NP * es, void*);
-  ALLOC(q, NP * es, void*);
-  ALLOC(partials, 2 * (size_t)s.ntiles * sizeof(double), double*);
-  ALLOC(counter, sizeof(unsigned), unsigned*);
-  e = dalloc(h, &p, (size_t)h->IE * h->JE * h->KE * sizeof(double));
-  if (e != hipSuccess) return map_err(e);
-  h->d_fort = static_cast<double*>(p);
-  e = dalloc(h, &p, sizeof(double));
-  if (e != hipSuccess) return map_err(e);
-  h->d_nb = static_cast<double*>(p);
-  e = dalloc(h, &p, sizeof(unsigned long long));
-  if (e != hipSuccess) return map_err(e);
-  h->d_smax = static_cast<unsigned long long*>(p);
+  ALLOC(smaxp, sizeof(unsigned long long), const double*);
 #undef ALLOC
-  s.nbp = h->d_nb;
-  s.smaxp = reinterpret_cast<const double*>(h->d_smax);
+  h->d_smax = reinterpret_cast<unsigned long long*>(const_cast<double*>(s.smaxp));
+  int arc = alloc_state(h);
+  if (arc != ORCTM_NH_OK) return arc;
   s.rtol = h->opts.rtol;
   s.rtol_inner = h->opts.rtol_inner;
   s.maxit = h->opts.maxit;
@@ -549,6 +593,8 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   orctm_nh_t* h = new (std::nothrow) orctm_nh_t();
   if (!h) return ORCTM_NH_ERR_NOMEM;
   h->opts = o;
+  h->op = std::make_shared<OpData>();
+  h->op->device = o.device;
   h->IE = grid->ie;
   h->JE = grid->je;
   h->KE = grid->ke;
@@ -584,9 +630,10 @@ int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg) {
   std::lock_guard<std::mutex> lk(h->mu);
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
-  hipError_t e = hipMemcpyAsync(h->d_fort, divg, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  hipError_t e = ensure_fort(h);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, divg, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
   if (e == hipSuccess)
-    e = launch_rhs(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
+    e = launch_rhs(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, h->d_fort, h->IE, h->JE);
   if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
   return map_err(e);
 }
@@ -596,7 +643,8 @@ int orctm_nh_set_guess(orctm_nh_t* h, const double* pnh) {
   std::lock_guard<std::mutex> lk(h->mu);
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
-  hipError_t e = hipMemcpyAsync(h->d_fort, pnh, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  hipError_t e = ensure_fort(h);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, pnh, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
   if (e == hipSuccess)
     e = launch_guess(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
   if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
@@ -659,6 +707,7 @@ int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
   hipError_t e = hipMalloc(&dx, bytes);
   if (e == hipSuccess) e = hipMalloc(&dy, bytes);
   if (e == hipSuccess) e = hipMemset(dx, 0, bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = to_device_vec(h, dx, x, sizeof(double));
   if (e == hipSuccess) e = launch_apply64(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dx, dy);
   if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
@@ -690,7 +739,7 @@ int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz) {
 
 int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes) {
   if (!h || !bytes) return ORCTM_NH_ERR_INVALID;
-  *bytes = (int64_t)h->bytes;
+  *bytes = (int64_t)(h->bytes + (h->op ? h->op->bytes : 0));
   return ORCTM_NH_OK;
 }
 
@@ -700,7 +749,9 @@ int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
   hipSetDevice(h->opts.device);
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
+  // x goes where the forward elimination leaves w; the matvec kernel in raw mode takes it as z
   hipError_t e = hipMemset(h->sys.z, 0, (size_t)h->plane * h->nz * es);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = to_device_vec(h, h->sys.z, x, es);
   if (e == hipSuccess) {
     if (h->fp32) {
@@ -717,7 +768,7 @@ int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
       e = hipMemcpy(c.d_inner, &in, sizeof(in), hipMemcpyHostToDevice);
     }
   }
-  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles);
+  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1, 1);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   if (e == hipSuccess) e = to_host_vec(h, y, h->sys.q, es);
   return map_err(e);
@@ -729,13 +780,16 @@ int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
   hipSetDevice(h->opts.device);
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
+  // forward elimination (init kernel) then back substitution inside the matvec kernel, whose first
+  // iteration writes p = z into search-direction buffer B
   Outer o{};
   o.odone = RUN;
   hipError_t e = hipMemcpy(c.d_outer, &o, sizeof(o), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = to_device_vec(h, h->sys.r, r, es);
-  if (e == hipSuccess) e = launch_init(c.stream, h->fp32, c.nzr, c.d_sys, c.d_tiles, c.ntiles);
+  if (e == hipSuccess) e = launch_init(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1);
+  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1, 0);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-  if (e == hipSuccess) e = to_host_vec(h, z, h->sys.z, es);
+  if (e == hipSuccess) e = to_host_vec(h, z, h->sys.pb, es);
   if (e == hipSuccess && rho) {
     if (h->fp32) {
       Inner<float> in;
@@ -747,6 +801,57 @@ int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
       *rho = in.rho;
     }
   }
+  return map_err(e);
+}
+
+int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
+  if (!src || !out) return ORCTM_NH_ERR_INVALID;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(src->mu);
+  hipSetDevice(src->opts.device);
+  orctm_nh_t* h = new (std::nothrow) orctm_nh_t();
+  if (!h) return ORCTM_NH_ERR_NOMEM;
+  h->opts = src->opts;
+  h->op = src->op;
+  h->IE = src->IE;
+  h->JE = src->JE;
+  h->KE = src->KE;
+  h->nx = src->nx;
+  h->ny = src->ny;
+  h->nz = src->nz;
+  h->pitch = src->pitch;
+  h->plane = src->plane;
+  h->fp32 = src->fp32;
+  h->tiles = src->tiles;
+  h->sys = src->sys;  // operator pointers shared; state pointers replaced below
+  h->d_smax = src->d_smax;
+  int rc = alloc_state(h);
+  hipError_t e = hipSuccess;
+  if (rc == ORCTM_NH_OK) {
+    const size_t NP = (size_t)h->plane * h->nz;
+    e = hipMemcpy(h->sys.b, src->sys.b, NP * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->sys.nbp, src->sys.nbp, sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess)
+      e = ctx_build(h->ctx, {h->sys}, {h->tiles}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
+                    h->opts.profile != 0);
+    rc = map_err(e);
+  }
+  if (rc != ORCTM_NH_OK) {
+    ctx_destroy(h->ctx);
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->opts.device);
+  hipError_t e = launch_perturb(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, amplitude, seed);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
   return map_err(e);
 }
 

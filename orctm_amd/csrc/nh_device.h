@@ -60,8 +60,7 @@ struct Sys {
   void* pa;             // search direction, buffer A
   void* pb;             // search direction, buffer B
   void* q;              // C p
-  double* partials;     // 2*ntiles slots of block partial sums
-  unsigned* counter;    // last-block counter
+  double* partials;     // 2*ntiles slots of block partial sums (summed by the reduction kernel)
   Outer* outer;
   void* inner;          // Inner<T>*
   double* nbp;          // ||b||_2 of the resident right-hand side (written by the rhs kernel)

@@ -19,23 +19,26 @@ hipError_t launch_assemble(hipStream_t st, bool fp32, const Sys* d_sys, const Ti
                            const FortranIn& f, int* d_bad, unsigned long long* d_smax_bits);
 hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
 // b = DIVG*WETO from a device copy of DIVG(IE,JE,KE); ||b|| into Outer.nb.
-hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_divg, int IE,
-                      int JE);
+hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                      const double* d_divg, int IE, int JE);
+// b <- b (1 + amplitude xi), xi ~ N(0,1) from (seed, cell); ||b|| refreshed.
+hipError_t launch_perturb(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                          double amplitude, unsigned long long seed);
 // Interior of a device Fortran (IE,JE,KE) array -> p64 (initial guess).
 hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_pnh, int IE,
                         int JE);
 // Per-system outer-state initialisation (nsys threads).
 hipError_t launch_begin(hipStream_t st, bool fp32, const Sys* d_sys, int nsys);
-hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
-hipError_t launch_init(hipStream_t st, bool fp32, int nzr, const Sys* d_sys, const Tile* d_tiles, int ntiles);
-hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
-hipError_t launch_update(hipStream_t st, bool fp32, int nzr, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+// Each phase launch below is followed by its one-block-per-system reduction (nsys blocks).
+hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
+hipError_t launch_init(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
+// raw = 1 (verification only): q = C w without the back substitution.
+hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                       int raw = 0);
+hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
 hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
 // y = A64 x on padded interior vectors (verification entry point).
 hipError_t launch_apply64(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* x,
                           double* y);
-
-// Register-array length used by the column kernels for a given nz (0 = global scratch path).
-int pick_nzr(bool fp32, int nz);
 
 }  // namespace onh

@@ -3,14 +3,16 @@
 // Every launch covers a list of 64x4-column tiles (Tile), possibly from several independent
 // systems (Sys); a block finds its system through its tile, so one launch advances a whole batch.
 // Reductions are deterministic: per-thread fma accumulation along k, a fixed wavefront butterfly
-// (lane 0's value), a fixed order over the 4 wavefronts, block partials summed in tile order by
-// the last block of each system (atomic ticket), so a system's result does not depend on what else
-// is in the launch.
+// (lane 0's value), a fixed order over the 4 wavefronts, block partials summed in tile order by a
+// one-block-per-system reduction kernel launched after each phase, so a system's result does not
+// depend on what else is in the launch.
 //
 // Floating-point steps that the CPU oracle restates bit for bit are written with explicit fma and
 // are compiled with -ffp-contract=off (no other contraction): the assembly of the reference
 // operator (mo_petsc.f90:543-782), the FP64 matvec, the inner matvec and the line preconditioner.
 #include <math.h>
+
+#include <type_traits>
 
 #include "nh_kernels.h"
 
@@ -18,6 +20,15 @@ namespace onh {
 
 __device__ __forceinline__ double fmad(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return fmaf(a, b, c); }
+
+// Pointers read from a Sys in memory are generic; casting them to the global address space lets the
+// compiler use global_load/global_store and count only vector-memory operations in its waits.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> G(const void* p) {
+  return (gptr<T>)(const_cast<void*>(p));
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -45,35 +56,16 @@ __device__ __forceinline__ void block_sum2(T& a, T& b) {
   }
 }
 
-// Publishes this block's (a, b) (valid in thread 0) and returns true, uniformly, in the last block of
-// the system to arrive; that block then holds the system totals in thread 0's a and b.
+// Publishes this block's (a, b) (valid in thread 0) into its system's partials slot. The system
+// totals are formed by k_reduce, launched right after, in tile order: no atomics, no fences (a
+// per-block device-scope fence costs about a third of the bandwidth of these streaming kernels).
 template <typename T>
-__device__ bool last_block2(const Sys& s, int local, T& a, T& b) {
-  __shared__ int am_last;
-  T* part = reinterpret_cast<T*>(s.partials);
+__device__ __forceinline__ void publish2(const Sys& s, int local, T a, T b) {
   if (threadIdx.x == 0) {
+    gptr<T> part = G<T>(s.partials);
     part[2 * local] = a;
     part[2 * local + 1] = b;
-    __threadfence();
-    const unsigned prev = atomicAdd(s.counter, 1u);
-    am_last = (prev == (unsigned)(s.ntiles - 1));
   }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-  const volatile T* vp = part;
-  T sa = T(0), sb = T(0);
-  for (int t = threadIdx.x; t < s.ntiles; t += NT) {
-    sa += vp[2 * t];
-    sb += vp[2 * t + 1];
-  }
-  block_sum2(sa, sb);
-  if (threadIdx.x == 0) {
-    *s.counter = 0u;
-    a = sa;
-    b = sb;
-  }
-  return true;
 }
 
 // Neighbour map of the DMDA: periodic wrap where cyclic, else validity (outside reads +0).
@@ -284,7 +276,42 @@ __global__ __launch_bounds__(NT) void k_rhs(const Sys* __restrict__ systems, con
     }
   }
   block_sum2(acc, dummy);
-  if (last_block2(s, t.local, acc, dummy) && threadIdx.x == 0) *s.nbp = sqrt(acc);
+  publish2(s, t.local, acc, dummy);
+}
+
+// b <- b * (1 + amplitude * xi), xi ~ N(0,1) per cell from a counter-based SplitMix64 stream
+// keyed by (seed, cell index); land cells stay 0. Synthetic time-step forcing for serving loads.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(NT) void k_perturb(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                double amplitude, unsigned long long seed) {
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
+  double acc = 0.0, dummy = 0.0;
+  if (i < s.nx && j < s.ny) {
+    gptr<double> b = G<double>(s.b);
+    const size_t col = (size_t)j * s.pitch + i;
+    const unsigned long long key = splitmix64(seed);
+    for (int k = 0; k < s.nz; ++k) {
+      const size_t idx = col + (size_t)k * s.plane;
+      const unsigned long long n = ((unsigned long long)k * s.ny + j) * s.nx + i;
+      const unsigned long long a = splitmix64(key + 2 * n), c = splitmix64(key + 2 * n + 1);
+      const double u1 = 1.0 - (double)(a >> 11) * (1.0 / 9007199254740992.0);
+      const double u2 = (double)(c >> 11) * (1.0 / 9007199254740992.0);
+      const double xi = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+      const double v = b[idx] * (1.0 + amplitude * xi);
+      b[idx] = v;
+      acc = fma(v, v, acc);
+    }
+  }
+  block_sum2(acc, dummy);
+  publish2(s, t.local, acc, dummy);
 }
 
 __global__ __launch_bounds__(NT) void k_guess(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
@@ -301,18 +328,25 @@ __global__ __launch_bounds__(NT) void k_guess(const Sys* __restrict__ systems, c
 // ------------------------------------------------------------------------------------------------
 // FP64 reference matvec, restated by orc_matvec64: centre, W, E, N, S, U, D, absent = +0.
 
-__device__ __forceinline__ double apply_a64(const Sys& s, const double* __restrict__ x, size_t NP, int i, int j, int k,
+__device__ __forceinline__ double apply_a64(const Sys& s, gptr<const double> x, size_t NP, int i, int j, int k,
                                             size_t idx) {
   const size_t IP = s.pitch, P = s.plane;
   const int nx = s.nx, ny = s.ny;
-  const double xw = i > 0 ? x[idx - 1] : (s.cyc_x ? x[idx + (size_t)(nx - 1)] : 0.0);
-  const double xe = i < nx - 1 ? x[idx + 1] : (s.cyc_x ? x[idx - (size_t)(nx - 1)] : 0.0);
-  const double xn = j > 0 ? x[idx - IP] : (s.cyc_y ? x[idx + IP * (size_t)(ny - 1)] : 0.0);
-  const double xs = j < ny - 1 ? x[idx + IP] : (s.cyc_y ? x[idx - IP * (size_t)(ny - 1)] : 0.0);
-  const double xu = k > 0 ? x[idx - P] : 0.0;
-  const double xd = k < s.nz - 1 ? x[idx + P] : 0.0;
-  const double* A = s.A64;
-  double acc = A[idx] * x[idx];
+  // neighbour indices with periodic wrap; absent neighbours read index idx and are replaced by +0
+  const bool hw = i > 0 || s.cyc_x, he = i < nx - 1 || s.cyc_x;
+  const bool hn = j > 0 || s.cyc_y, hs = j < ny - 1 || s.cyc_y;
+  const size_t iw = i > 0 ? idx - 1 : (s.cyc_x ? idx + (size_t)(nx - 1) : idx);
+  const size_t ie = i < nx - 1 ? idx + 1 : (s.cyc_x ? idx - (size_t)(nx - 1) : idx);
+  const size_t in_ = j > 0 ? idx - IP : (s.cyc_y ? idx + IP * (size_t)(ny - 1) : idx);
+  const size_t is = j < ny - 1 ? idx + IP : (s.cyc_y ? idx - IP * (size_t)(ny - 1) : idx);
+  const size_t iu = k > 0 ? idx - P : idx;
+  const size_t id = k < s.nz - 1 ? idx + P : idx;
+  const double x0 = x[idx];
+  const double vw = x[iw], ve = x[ie], vn = x[in_], vs = x[is], vu = x[iu], vd = x[id];
+  const double xw = hw ? vw : 0.0, xe = he ? ve : 0.0, xn = hn ? vn : 0.0, xs = hs ? vs : 0.0;
+  const double xu = k > 0 ? vu : 0.0, xd = k < s.nz - 1 ? vd : 0.0;
+  gptr<const double> A = G<const double>(s.A64);
+  double acc = A[idx] * x0;
   acc = fma(A[NP + idx], xw, acc);
   acc = fma(A[2 * NP + idx], xe, acc);
   acc = fma(A[3 * NP + idx], xn, acc);
@@ -331,7 +365,7 @@ __global__ __launch_bounds__(NT) void k_apply64(const Sys* __restrict__ systems,
   const size_t NP = (size_t)s.plane * s.nz, col = (size_t)j * s.pitch + i;
   for (int k = 0; k < s.nz; ++k) {
     const size_t idx = col + (size_t)k * s.plane;
-    y[idx] = apply_a64(s, x, NP, i, j, k, idx);
+    y[idx] = apply_a64(s, G<const double>(x), NP, i, j, k, idx);
   }
 }
 
@@ -375,7 +409,8 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
   double acc = 0.0, dummy = 0.0;
   if (i < s.nx && j < s.ny) {
     const size_t NP = (size_t)s.plane * s.nz, col = (size_t)j * s.pitch + i;
-    T* rin = reinterpret_cast<T*>(s.r);
+    gptr<T> rin = G<T>(s.r);
+#pragma unroll 2
     for (int k = 0; k < s.nz; ++k) {
       const size_t idx = col + (size_t)k * s.plane;
       if (zero) {
@@ -383,41 +418,13 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         rin[idx] = T(0);
         continue;
       }
-      const double r = s.b[idx] - apply_a64(s, s.p64, NP, i, j, k, idx);
+      const double r = G<const double>(s.b)[idx] - apply_a64(s, G<const double>(s.p64), NP, i, j, k, idx);
       acc = fma(r, r, acc);
-      rin[idx] = (T)((s.S[idx] * (-r)) * gamma);
+      rin[idx] = (T)((G<const double>(s.S)[idx] * (-r)) * gamma);
     }
   }
   block_sum2(acc, dummy);
-  if (last_block2(s, t.local, acc, dummy) && threadIdx.x == 0) {
-    const double nr = sqrt(acc);
-    o->nr = nr;
-    int od = RUN;
-    if (zero) {
-      o->rel = 0.0;
-      od = CONVERGED;
-    } else {
-      const double rel = nr / nb;
-      o->rel = rel;
-      if (!isfinite(rel))
-        od = BREAKDOWN;
-      else if (rel <= o->rtol)
-        od = CONVERGED;
-      else if (o->outer >= o->max_outer || o->inner_total >= o->maxit)
-        od = NOTCONV;
-      else {
-        double tol = 0.5 * o->rtol * nb / nr;
-        if (tol < o->rtol_inner) tol = o->rtol_inner;
-        in->tol2 = (T)(tol * tol);
-        in->maxit = o->maxit - o->inner_total;
-        o->inv_gamma = o->smax * o->nr_prev;
-        o->nr_prev = nr;
-        o->outer += 1;
-      }
-    }
-    o->odone = od;
-    if (od != RUN) in->done = SKIP;
-  }
+  publish2(s, t.local, acc, dummy);
 }
 
 template <typename T>
@@ -429,19 +436,30 @@ __global__ __launch_bounds__(NT) void k_refine(const Sys* __restrict__ systems, 
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   if (i >= s.nx || j >= s.ny) return;
   const double g = o->inv_gamma;
-  const T* x = reinterpret_cast<const T*>(s.x);
+  gptr<const T> x = G<const T>(s.x);
+  gptr<double> p64 = G<double>(s.p64);
   const size_t col = (size_t)j * s.pitch + i;
+#pragma unroll 4
   for (int k = 0; k < s.nz; ++k) {
     const size_t idx = col + (size_t)k * s.plane;
-    s.p64[idx] = s.p64[idx] + (double)x[idx] * g;
+    p64[idx] = p64[idx] + (double)x[idx] * g;
   }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Inner PCG. Column kernel (init / update): x, r update, line solve z = M^{-1} r, r.z and r.r.
+// Inner PCG, two kernels per iteration.
+//
+//   k_fwd  (per column, top-down):   x += alpha p; r -= alpha q; forward elimination of the line
+//          preconditioner  y_k = r_k - l_k y_{k-1},  w_k = y_k / D_k;  r.z = sum y_k w_k,  r.r.
+//          Reads x p r q l invD, writes x r w: 9 words per cell.
+//   k_bwd  (64x4 tile, bottom-up):   back substitution z_k = w_k - l_{k+1} z_{k+1} fused with
+//          p = z + beta p_old, q = C p (7-point, horizontal neighbours through LDS) and p.q.
+//          Reads w l p_old + 7 coefficients, writes p q: 12 words per cell.
+// The halo lanes of k_bwd run the back-substitution recurrence of their halo columns themselves, so
+// z never goes to memory and no grid-wide step separates the two halves of the line solve.
 
-template <typename T, int NZR, bool INIT>
-__global__ __launch_bounds__(NT) void k_column(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
+template <typename T, bool INIT>
+__global__ __launch_bounds__(NT) void k_fwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
@@ -454,133 +472,79 @@ __global__ __launch_bounds__(NT) void k_column(const Sys* __restrict__ systems, 
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   const size_t P = s.plane, col = (size_t)j * s.pitch + i;
   const int nz = s.nz;
-  T* __restrict__ x = reinterpret_cast<T*>(s.x);
-  T* __restrict__ r = reinterpret_cast<T*>(s.r);
-  T* __restrict__ z = reinterpret_cast<T*>(s.z);
-  const T* __restrict__ q = reinterpret_cast<const T*>(s.q);
-  const T* __restrict__ p = reinterpret_cast<const T*>(in->pbuf ? s.pa : s.pb);
-  const T* __restrict__ L = reinterpret_cast<const T*>(s.L);
-  const T* __restrict__ invD = reinterpret_cast<const T*>(s.invD);
+  gptr<T> x = G<T>(s.x);
+  gptr<T> r = G<T>(s.r);
+  gptr<T> w = G<T>(s.z);
+  gptr<const T> q = G<const T>(s.q);
+  gptr<const T> p = G<const T>(in->pbuf ? s.pa : s.pb);
+  gptr<const T> L = G<const T>(s.L);
+  gptr<const T> invD = G<const T>(s.invD);
   const T alpha = INIT ? T(0) : in->alpha;
   T rho = T(0), rr = T(0);
   if (i < s.nx && j < s.ny) {
-    T yprev = T(0);
-    if constexpr (NZR > 0) {
-      T wreg[NZR];
+    // Batches of U levels in two register sets (ping-pong): the loads of batch b+1 are issued before
+    // batch b is computed and stored, so stores (which may alias later loads as far as the compiler
+    // knows) never sit between a load and its use, and no prefetched value has to be moved.
+    constexpr int U = 4;
+    struct Batch {
+      T x[U], p[U], r[U], q[U], l[U], d[U];
+    };
+    Batch b0, b1;
+    auto load = [&](int k0, Batch& b) {
 #pragma unroll
-      for (int k = 0; k < NZR; ++k) {
-        if (k < nz) {
-          const size_t idx = col + (size_t)k * P;
-          T rv;
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u < nz ? k0 + u : nz - 1;
+        const size_t idx = col + (size_t)k * P;
+        b.r[u] = r[idx];
+        b.l[u] = L[idx];
+        b.d[u] = invD[idx];
+        if (!INIT) {
+          b.x[u] = x[idx];
+          b.p[u] = p[idx];
+          b.q[u] = q[idx];
+        }
+      }
+    };
+    T yprev = T(0);
+    auto compute = [&](int k0, const Batch& b) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k0 + u < nz) {
+          const size_t idx = col + (size_t)(k0 + u) * P;
+          T rk = b.r[u];
           if (INIT) {
-            rv = r[idx];
             x[idx] = T(0);
           } else {
-            T xv = x[idx];
-            rv = r[idx];
-            xv = fmad(alpha, p[idx], xv);
-            rv = fmad(-alpha, q[idx], rv);
-            x[idx] = xv;
-            r[idx] = rv;
+            x[idx] = fmad(alpha, b.p[u], b.x[u]);
+            rk = fmad(-alpha, b.q[u], rk);
+            r[idx] = rk;
           }
-          rr = fmad(rv, rv, rr);
-          const T y = fmad(-L[idx], yprev, rv);
-          const T w = y * invD[idx];
-          rho = fmad(y, w, rho);
-          wreg[k] = w;
+          rr = fmad(rk, rk, rr);
+          const T y = fmad(-b.l[u], yprev, rk);
+          const T wv = y * b.d[u];
+          rho = fmad(y, wv, rho);
+          w[idx] = wv;
           yprev = y;
         }
       }
-      T zn = T(0);
-#pragma unroll
-      for (int k = NZR - 1; k >= 0; --k) {
-        if (k < nz) {
-          const size_t idx = col + (size_t)k * P;
-          T zk;
-          if (k == nz - 1)
-            zk = wreg[k];
-          else
-            zk = fmad(-L[idx + P], zn, wreg[k]);
-          z[idx] = zk;
-          zn = zk;
-        }
-      }
-    } else {
-      for (int k = 0; k < nz; ++k) {
-        const size_t idx = col + (size_t)k * P;
-        T rv;
-        if (INIT) {
-          rv = r[idx];
-          x[idx] = T(0);
-        } else {
-          T xv = x[idx];
-          rv = r[idx];
-          xv = fmad(alpha, p[idx], xv);
-          rv = fmad(-alpha, q[idx], rv);
-          x[idx] = xv;
-          r[idx] = rv;
-        }
-        rr = fmad(rv, rv, rr);
-        const T y = fmad(-L[idx], yprev, rv);
-        const T w = y * invD[idx];
-        rho = fmad(y, w, rho);
-        z[idx] = w;
-        yprev = y;
-      }
-      T zn = z[col + (size_t)(nz - 1) * P];
-      for (int k = nz - 2; k >= 0; --k) {
-        const size_t idx = col + (size_t)k * P;
-        const T zk = fmad(-L[idx + P], zn, z[idx]);
-        z[idx] = zk;
-        zn = zk;
-      }
+    };
+    load(0, b0);
+    for (int k0 = 0; k0 < nz; k0 += 2 * U) {
+      load(k0 + U, b1);
+      compute(k0, b0);
+      if (k0 + U >= nz) break;
+      load(k0 + 2 * U, b0);
+      compute(k0 + U, b1);
     }
   }
   block_sum2(rho, rr);
-  if (last_block2(s, t.local, rho, rr) && threadIdx.x == 0) {
-    if (INIT) {
-      in->rho = rho;
-      in->rr0 = rr;
-      in->rr = rr;
-      in->iter = 0;
-      in->first = 1;
-      in->pbuf = 0;
-      in->beta = T(0);
-      if (rr == T(0))
-        in->done = CONVERGED;
-      else if (!(rho > T(0)) || !isfinite((double)rho) || !isfinite((double)rr))
-        in->done = BREAKDOWN;
-      else
-        in->done = RUN;
-      if (in->done == BREAKDOWN) o->odone = BREAKDOWN;
-    } else {
-      in->iter += 1;
-      o->inner_total += 1;
-      in->rr = rr;
-      int d = RUN;
-      if (rr <= in->tol2 * in->rr0)
-        d = CONVERGED;
-      else if (!(rho > T(0)) || !isfinite((double)rho) || !isfinite((double)rr))
-        d = BREAKDOWN;
-      else if (in->iter >= in->maxit)
-        d = NOTCONV;
-      else {
-        in->beta = rho / in->rho;
-        in->rho = rho;
-      }
-      in->first = 0;
-      in->pbuf ^= 1;
-      in->done = d;
-    }
-  }
+  publish2(s, t.local, rho, rr);
 }
 
-// Matvec kernel: p_new = z + beta p_old (or z on the first iteration), q = C p_new, p.q.
-// 64x4 tile in LDS, k-marching with p(k-1), p(k), p(k+1) in registers; three LDS planes so one
-// barrier per level suffices. Halo: wave 0 the north row, wave 1 the south row, wave 2 the west
-// and east columns.
+// raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_spmv(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
+__global__ __launch_bounds__(NT) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                            int raw) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
@@ -592,13 +556,15 @@ __global__ __launch_bounds__(NT) void k_spmv(const Sys* __restrict__ systems, co
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const bool active = i < nx && j < ny;
   const size_t P = s.plane, IP = s.pitch, NP = P * (size_t)nz;
-  const int first = in->first, pb = in->pbuf;
+  const bool first = in->first != 0;
+  const int pb = in->pbuf;
   const T beta = in->beta;
-  const T* __restrict__ zv = reinterpret_cast<const T*>(s.z);
-  const T* __restrict__ pold = reinterpret_cast<const T*>(pb ? s.pb : s.pa);
-  T* __restrict__ pnew = reinterpret_cast<T*>(pb ? s.pa : s.pb);
-  T* __restrict__ q = reinterpret_cast<T*>(s.q);
-  const T* __restrict__ C = reinterpret_cast<const T*>(s.C);
+  gptr<const T> w = G<const T>(s.z);
+  gptr<const T> L = G<const T>(s.L);
+  gptr<const T> pold = G<const T>(pb ? s.pb : s.pa);
+  gptr<T> pnew = G<T>(pb ? s.pa : s.pb);
+  gptr<T> q = G<T>(s.q);
+  gptr<const T> C = G<const T>(s.C);
 
   __shared__ T lds[3][TY + 2][TX + 2];
 
@@ -633,55 +599,179 @@ __global__ __launch_bounds__(NT) void k_spmv(const Sys* __restrict__ systems, co
   const bool hval = has_halo && map_ij(hi, hj, nx, ny, s.cyc_x, s.cyc_y);
   const size_t hoff = hval ? (size_t)hj * IP + hi : 0;
 
-  auto pv = [&](size_t idx) -> T { return first ? zv[idx] : fmad(beta, pold[idx], zv[idx]); };
-
-  T c0 = cval ? pv(coff) : T(0);
-  lds[0][ty + 1][tx + 1] = c0;
-  if (has_halo) lds[0][hy][hx] = hval ? pv(hoff) : T(0);
-  if (active) pnew[coff] = c0;
-  T pm1 = T(0), p0 = c0, dot = T(0);
-  __syncthreads();
-  for (int k = 0; k < nz; ++k) {
-    const size_t ck = (size_t)k * P + coff;
-    T cc = T(0), cw = T(0), ce = T(0), cn = T(0), cs = T(0), cu = T(0), cd = T(0);
-    if (active) {
-      cc = C[ck];
-      cw = C[NP + ck];
-      ce = C[2 * NP + ck];
-      cn = C[3 * NP + ck];
-      cs = C[4 * NP + ck];
-      cu = C[5 * NP + ck];
-      cd = C[6 * NP + ck];
+  // Software pipeline over levels (bottom-up) in two register sets used alternately: while level k
+  // is finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its
+  // coefficients), set `nxt` receives w/l/p_old of level k-2 and the coefficients of level k-1.
+  struct Stage {
+    T w, l, p, wh, lh, ph;
+    T c[NCOEF];
+  };
+  T zc = T(0), lc = T(0), zh = T(0), lh = T(0);  // z and l of the level below (centre, halo)
+  auto load_a = [&](int m, Stage& st) {
+    const size_t om = (size_t)(m > 0 ? m : 0) * P;
+    st.w = w[om + coff];
+    st.l = L[om + coff];
+    st.p = pold[om + coff];
+    if (has_halo) {
+      st.wh = w[om + hoff];
+      st.lh = L[om + hoff];
+      st.ph = pold[om + hoff];
     }
-    T pp1 = T(0);
-    if (k + 1 < nz) {
-      const int b1 = (k + 1) % 3;
-      const size_t o1 = (size_t)(k + 1) * P;
-      pp1 = cval ? pv(o1 + coff) : T(0);
-      lds[b1][ty + 1][tx + 1] = pp1;
-      if (has_halo) lds[b1][hy][hx] = hval ? pv(o1 + hoff) : T(0);
-      if (active) pnew[o1 + coff] = pp1;
+  };
+  auto load_c = [&](int m, Stage& st) {
+    const size_t cm = (size_t)(m > 0 ? m : 0) * P + coff;
+#pragma unroll
+    for (int c = 0; c < NCOEF; ++c) st.c[c] = C[c * NP + cm];
+  };
+  // back substitution and search-direction update of level m from w, l, p_old (centre and halo)
+  auto level = [&](int m, T wc, T Lc, T pc, T wh, T Lh, T ph) -> T {
+    const int buf = m % 3;
+    const size_t om = (size_t)m * P;
+    const T zm = fmad(-lc, zc, wc);
+    const T pm = first ? zm : fmad(beta, pc, zm);
+    zc = zm;
+    lc = raw ? T(0) : Lc;
+    const T pout = cval ? pm : T(0);
+    lds[buf][ty + 1][tx + 1] = pout;
+    if (active) pnew[om + coff] = pm;
+    if (has_halo) {
+      const T zmh = fmad(-lh, zh, wh);
+      const T pmh = first ? zmh : fmad(beta, ph, zmh);
+      zh = zmh;
+      lh = raw ? T(0) : Lh;
+      lds[buf][hy][hx] = hval ? pmh : T(0);
     }
+    return pout;
+  };
+  T p0 = T(0), pp1 = T(0), dot = T(0);
+  auto step = [&](int k, Stage& cur, Stage& nxt) {
+    load_a(k - 2, nxt);
+    load_c(k - 1, nxt);
+    const T pm1 = k > 0 ? level(k - 1, cur.w, cur.l, cur.p, cur.wh, cur.lh, cur.ph) : T(0);
     __syncthreads();
+    const int b0 = k % 3;
+    T acc = cur.c[0] * p0;
+    acc = fmad(cur.c[1], lds[b0][ty + 1][tx], acc);
+    acc = fmad(cur.c[2], lds[b0][ty + 1][tx + 2], acc);
+    acc = fmad(cur.c[3], lds[b0][ty][tx + 1], acc);
+    acc = fmad(cur.c[4], lds[b0][ty + 2][tx + 1], acc);
+    acc = fmad(cur.c[5], pm1, acc);
+    acc = fmad(cur.c[6], pp1, acc);
     if (active) {
-      const int b0 = k % 3;
-      T acc = cc * p0;
-      acc = fmad(cw, lds[b0][ty + 1][tx], acc);
-      acc = fmad(ce, lds[b0][ty + 1][tx + 2], acc);
-      acc = fmad(cn, lds[b0][ty][tx + 1], acc);
-      acc = fmad(cs, lds[b0][ty + 2][tx + 1], acc);
-      acc = fmad(cu, pm1, acc);
-      acc = fmad(cd, pp1, acc);
-      q[ck] = acc;
+      q[(size_t)k * P + coff] = acc;
       dot = fmad(p0, acc, dot);
     }
-    pm1 = p0;
-    p0 = pp1;
+    pp1 = p0;
+    p0 = pm1;
+  };
+  Stage s0, s1;
+  load_a(nz - 1, s1);
+  load_a(nz - 2, s0);
+  load_c(nz - 1, s0);
+  p0 = level(nz - 1, s1.w, s1.l, s1.p, s1.wh, s1.lh, s1.ph);
+  __syncthreads();
+  for (int k = nz - 1; k >= 0; k -= 2) {
+    step(k, s0, s1);
+    if (k == 0) break;
+    step(k - 1, s1, s0);
   }
   T dummy = T(0);
   block_sum2(dot, dummy);
-  if (last_block2(s, t.local, dot, dummy) && threadIdx.x == 0) {
-    const T pq = dot;
+  publish2(s, t.local, dot, dummy);
+}
+
+// ------------------------------------------------------------------------------------------------
+// System totals and scalar recurrences: one block per system, partials summed in tile order.
+
+enum RedMode { RED_NB = 0, RED_RES = 1, RED_INIT = 2, RED_FWD = 3, RED_BWD = 4 };
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(NT) void k_reduce(const Sys* __restrict__ systems) {
+  const Sys& s = systems[blockIdx.x];
+  Outer* o = s.outer;
+  Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
+  if ((MODE == RED_RES || MODE == RED_INIT) && o->odone != RUN) return;
+  if ((MODE == RED_FWD || MODE == RED_BWD) && in->done != RUN) return;
+  // the rhs and residual kernels publish FP64 partials; the inner kernels publish T partials
+  using Acc = typename std::conditional<(MODE == RED_NB || MODE == RED_RES), double, T>::type;
+  gptr<const Acc> part = G<const Acc>(s.partials);
+  Acc a = Acc(0), b = Acc(0);
+  for (int t = threadIdx.x; t < s.ntiles; t += NT) {
+    a += part[2 * t];
+    b += part[2 * t + 1];
+  }
+  block_sum2(a, b);
+  if (threadIdx.x != 0) return;
+  if (MODE == RED_NB) {
+    *s.nbp = sqrt((double)a);
+  } else if (MODE == RED_RES) {
+    // FP64 residual norm: convergence test and set-up of the next inner solve
+    const double nb = o->nb;
+    const bool zero = (nb == 0.0);
+    const double nr = sqrt((double)a);
+    o->nr = nr;
+    int od = RUN;
+    if (zero) {
+      o->rel = 0.0;
+      od = CONVERGED;
+    } else {
+      const double rel = nr / nb;
+      o->rel = rel;
+      if (!isfinite(rel))
+        od = BREAKDOWN;
+      else if (rel <= o->rtol)
+        od = CONVERGED;
+      else if (o->outer >= o->max_outer || o->inner_total >= o->maxit)
+        od = NOTCONV;
+      else {
+        double tol = 0.5 * o->rtol * nb / nr;
+        if (tol < o->rtol_inner) tol = o->rtol_inner;
+        in->tol2 = (T)(tol * tol);
+        in->maxit = o->maxit - o->inner_total;
+        o->inv_gamma = o->smax * o->nr_prev;
+        o->nr_prev = nr;
+        o->outer += 1;
+      }
+    }
+    o->odone = od;
+    if (od != RUN) in->done = SKIP;
+  } else if (MODE == RED_INIT) {
+    const T rho = a, rr = b;
+    in->rho = rho;
+    in->rr0 = rr;
+    in->rr = rr;
+    in->iter = 0;
+    in->first = 1;
+    in->pbuf = 0;
+    in->beta = T(0);
+    if (rr == T(0))
+      in->done = CONVERGED;
+    else if (!(rho > T(0)) || !isfinite((double)rho) || !isfinite((double)rr))
+      in->done = BREAKDOWN;
+    else
+      in->done = RUN;
+    if (in->done == BREAKDOWN) o->odone = BREAKDOWN;
+  } else if (MODE == RED_FWD) {
+    const T rho = a, rr = b;
+    in->iter += 1;
+    o->inner_total += 1;
+    in->rr = rr;
+    int d = RUN;
+    if (rr <= in->tol2 * in->rr0)
+      d = CONVERGED;
+    else if (!(rho > T(0)) || !isfinite((double)rho) || !isfinite((double)rr))
+      d = BREAKDOWN;
+    else if (in->iter >= in->maxit)
+      d = NOTCONV;
+    else {
+      in->beta = rho / in->rho;
+      in->rho = rho;
+    }
+    in->first = 0;
+    in->pbuf ^= 1;
+    in->done = d;
+  } else {  // RED_BWD
+    const T pq = a;
     const T alpha = in->rho / pq;
     in->pq = pq;
     in->alpha = alpha;
@@ -694,13 +784,6 @@ __global__ __launch_bounds__(NT) void k_spmv(const Sys* __restrict__ systems, co
 
 // ------------------------------------------------------------------------------------------------
 // Launchers.
-
-int pick_nzr(bool fp32, int nz) {
-  if (nz <= 32) return 32;
-  if (nz <= 64) return 64;
-  if (fp32 && nz <= 128) return 128;
-  return 0;
-}
 
 #define LAUNCH(kern, grid, st, ...)                                \
   do {                                                             \
@@ -720,9 +803,29 @@ hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const
   LAUNCH(k_line_factor<double>, ntiles, st, d_sys, d_tiles);
 }
 
-hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_divg, int IE,
-                      int JE) {
-  LAUNCH(k_rhs, ntiles, st, d_sys, d_tiles, d_divg, IE, JE);
+template <typename T, int MODE>
+static hipError_t launch_reduce(hipStream_t st, const Sys* d_sys, int nsys) {
+  hipLaunchKernelGGL((k_reduce<T, MODE>), dim3(nsys), dim3(NT), 0, st, d_sys);
+  return hipGetLastError();
+}
+
+#define PHASE(kern, red, grid, st, ...)                                        \
+  do {                                                                         \
+    if ((grid) <= 0) return hipSuccess;                                        \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, st, __VA_ARGS__);         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) return e_;                                           \
+    return red;                                                                \
+  } while (0)
+
+hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                      const double* d_divg, int IE, int JE) {
+  PHASE(k_rhs, (launch_reduce<double, RED_NB>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, d_divg, IE, JE);
+}
+
+hipError_t launch_perturb(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                          double amplitude, unsigned long long seed) {
+  PHASE(k_perturb, (launch_reduce<double, RED_NB>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, amplitude, seed);
 }
 
 hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_pnh, int IE,
@@ -739,9 +842,9 @@ hipError_t launch_begin(hipStream_t st, bool fp32, const Sys* d_sys, int nsys) {
   return hipGetLastError();
 }
 
-hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
-  if (fp32) LAUNCH(k_residual<float>, ntiles, st, d_sys, d_tiles);
-  LAUNCH(k_residual<double>, ntiles, st, d_sys, d_tiles);
+hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys) {
+  if (fp32) PHASE(k_residual<float>, (launch_reduce<float, RED_RES>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
+  PHASE(k_residual<double>, (launch_reduce<double, RED_RES>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
 }
 
 hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
@@ -749,35 +852,22 @@ hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
   LAUNCH(k_refine<double>, ntiles, st, d_sys, d_tiles);
 }
 
-template <bool INIT>
-static hipError_t launch_column(hipStream_t st, bool fp32, int nzr, const Sys* d_sys, const Tile* d_tiles,
-                                int ntiles) {
-  if (fp32) {
-    switch (nzr) {
-      case 32: LAUNCH((k_column<float, 32, INIT>), ntiles, st, d_sys, d_tiles);
-      case 64: LAUNCH((k_column<float, 64, INIT>), ntiles, st, d_sys, d_tiles);
-      case 128: LAUNCH((k_column<float, 128, INIT>), ntiles, st, d_sys, d_tiles);
-      default: LAUNCH((k_column<float, 0, INIT>), ntiles, st, d_sys, d_tiles);
-    }
-  }
-  switch (nzr) {
-    case 32: LAUNCH((k_column<double, 32, INIT>), ntiles, st, d_sys, d_tiles);
-    case 64: LAUNCH((k_column<double, 64, INIT>), ntiles, st, d_sys, d_tiles);
-    default: LAUNCH((k_column<double, 0, INIT>), ntiles, st, d_sys, d_tiles);
-  }
+hipError_t launch_init(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys) {
+  if (fp32)
+    PHASE((k_fwd<float, true>), (launch_reduce<float, RED_INIT>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
+  PHASE((k_fwd<double, true>), (launch_reduce<double, RED_INIT>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
 }
 
-hipError_t launch_init(hipStream_t st, bool fp32, int nzr, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
-  return launch_column<true>(st, fp32, nzr, d_sys, d_tiles, ntiles);
+hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys) {
+  if (fp32)
+    PHASE((k_fwd<float, false>), (launch_reduce<float, RED_FWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
+  PHASE((k_fwd<double, false>), (launch_reduce<double, RED_FWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
 }
 
-hipError_t launch_update(hipStream_t st, bool fp32, int nzr, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
-  return launch_column<false>(st, fp32, nzr, d_sys, d_tiles, ntiles);
-}
-
-hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
-  if (fp32) LAUNCH(k_spmv<float>, ntiles, st, d_sys, d_tiles);
-  LAUNCH(k_spmv<double>, ntiles, st, d_sys, d_tiles);
+hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
+                       int raw) {
+  if (fp32) PHASE(k_bwd<float>, (launch_reduce<float, RED_BWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, raw);
+  PHASE(k_bwd<double>, (launch_reduce<double, RED_BWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, raw);
 }
 
 hipError_t launch_apply64(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* x,

@@ -1,0 +1,109 @@
+! orctm_nh_c.f90 -- ISO_C_BINDING interface to liborctm_nh.so (include/orctm_nh.h) and a drop-in
+! replacement for the three MO_PETSC entry points of ORCTM (ORCTM/src_orctm/mo_petsc.f90):
+!
+!   petsc_start     (mo_petsc.f90:41-128, called from orctm.f90:970)
+!   petsc_solve_3D  (mo_petsc.f90:211-233, called from ocvtot_nh.f90:150)
+!   petsc_end       (mo_petsc.f90:131-143, called from orctm.f90:1585,1651)
+!
+! In ocvtot_nh.f90 and orctm.f90 replace `use mo_petsc` by `use orctm_nh_c`; nothing else changes.
+! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
+! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
+! One GPU holds the whole domain (nprocx = nprocy = 1). Build: gfortran/ifort -c orctm_nh_c.f90 and
+! link -L<repo>/orctm_amd -lorctm_nh.
+MODULE orctm_nh_c
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+  PRIVATE
+  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end
+  PUBLIC :: orctm_nh_grid_t, orctm_nh_options_t, orctm_nh_stats_t
+
+  INTEGER(C_INT), PARAMETER, PUBLIC :: ORCTM_NH_OK = 0, ORCTM_NH_FP64 = 0, ORCTM_NH_MIXED = 1
+
+  TYPE, BIND(C) :: orctm_nh_grid_t
+    INTEGER(C_INT32_T) :: ie, je, ke, i_start, j_start
+    TYPE(C_PTR) :: dlxp, dlyp, dlxu, dlyv, ddpo, weto, dzw
+    INTEGER(C_INT32_T) :: cyclic_x, cyclic_y, open_w, open_e, open_n, open_s
+  END TYPE
+
+  TYPE, BIND(C) :: orctm_nh_options_t
+    INTEGER(C_INT32_T) :: precision, maxit, max_outer, warm_start
+    REAL(C_DOUBLE) :: rtol, rtol_inner
+    INTEGER(C_INT32_T) :: check_every, device, profile
+    INTEGER(C_INT32_T) :: reserved(5)
+  END TYPE
+
+  TYPE, BIND(C) :: orctm_nh_stats_t
+    INTEGER(C_INT32_T) :: status, outer_iters, inner_iters, reserved0
+    REAL(C_DOUBLE) :: rel_res, bnorm, solve_ms, total_ms
+    REAL(C_DOUBLE) :: reserved(4)
+  END TYPE
+
+  INTERFACE
+    SUBROUTINE orctm_nh_default_options(opts) BIND(C, name="orctm_nh_default_options")
+      IMPORT :: orctm_nh_options_t
+      TYPE(orctm_nh_options_t), INTENT(OUT) :: opts
+    END SUBROUTINE
+    INTEGER(C_INT) FUNCTION orctm_nh_create(grid, opts, handle) BIND(C, name="orctm_nh_create")
+      IMPORT :: C_INT, C_PTR, orctm_nh_grid_t, orctm_nh_options_t
+      TYPE(orctm_nh_grid_t), INTENT(IN) :: grid
+      TYPE(orctm_nh_options_t), INTENT(IN) :: opts
+      TYPE(C_PTR), INTENT(OUT) :: handle
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_solve(handle, divg, pnh, stats) BIND(C, name="orctm_nh_solve")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, orctm_nh_stats_t
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(IN) :: divg(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: pnh(*)
+      TYPE(orctm_nh_stats_t), INTENT(OUT) :: stats
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_destroy(handle) BIND(C, name="orctm_nh_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+    END FUNCTION
+  END INTERFACE
+
+  TYPE(C_PTR), SAVE :: nh_handle = C_NULL_PTR
+
+CONTAINS
+
+  ! mo_petsc.f90:41-128 -- sizes, metrics and masks come from the module globals, as in compute_matrix.
+  SUBROUTINE petsc_start
+    USE MO_PARAM1, ONLY: IE, JE, KE, ICYCLI_X, ICYCLI_Y, I_start, J_start
+    USE MO_COMMO1, ONLY: DLXP, DLYP, DLXU, DLYV, DDPO, WETO, DZW
+    USE MO_OBCS, ONLY: West, East, North, South
+    TYPE(orctm_nh_grid_t) :: g
+    TYPE(orctm_nh_options_t) :: o
+    INTEGER(C_INT) :: ierr
+    g%ie = IE; g%je = JE; g%ke = KE
+    g%i_start = I_start; g%j_start = J_start
+    g%dlxp = C_LOC(DLXP); g%dlyp = C_LOC(DLYP)
+    g%dlxu = C_LOC(DLXU); g%dlyv = C_LOC(DLYV)
+    g%ddpo = C_LOC(DDPO); g%weto = C_LOC(WETO); g%dzw = C_LOC(DZW)
+    g%cyclic_x = ICYCLI_X; g%cyclic_y = ICYCLI_Y
+    g%open_w = MERGE(1, 0, West);  g%open_e = MERGE(1, 0, East)
+    g%open_n = MERGE(1, 0, North); g%open_s = MERGE(1, 0, South)
+    CALL orctm_nh_default_options(o)
+    ierr = orctm_nh_create(g, o, nh_handle)
+    IF (ierr /= ORCTM_NH_OK) THEN
+      PRINT *, 'orctm_nh_create failed, status ', ierr
+      STOP
+    ENDIF
+  END SUBROUTINE petsc_start
+
+  ! mo_petsc.f90:211-233 -- reads DIVG, writes the PNH interior, warm-starts from the last solution.
+  SUBROUTINE petsc_solve_3D
+    USE MO_COMMO1, ONLY: DIVG, PNH
+    TYPE(orctm_nh_stats_t) :: st
+    INTEGER(C_INT) :: ierr
+    ierr = orctm_nh_solve(nh_handle, DIVG, PNH, st)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_solve status ', ierr, ' rel.res ', st%rel_res
+  END SUBROUTINE petsc_solve_3D
+
+  ! mo_petsc.f90:131-143
+  SUBROUTINE petsc_end
+    INTEGER(C_INT) :: ierr
+    IF (C_ASSOCIATED(nh_handle)) ierr = orctm_nh_destroy(nh_handle)
+    nh_handle = C_NULL_PTR
+  END SUBROUTINE petsc_end
+
+END MODULE orctm_nh_c

@@ -55,17 +55,28 @@ def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optiona
 class NHSolver:
     """One system on one GPU: petsc_start on construction, petsc_solve_3D per time step."""
 
-    def __init__(self, grid, opts: Optional[abi.OptionsC] = None, **kw):
+    def __init__(self, grid, opts: Optional[abi.OptionsC] = None, _handle=None, **kw):
         self.lib = abi.load()
         self.grid = grid
         self.opts = opts if opts is not None else options(**kw)
-        h = ctypes.c_void_p()
-        rc = self.lib.orctm_nh_create(ctypes.byref(grid.c_struct()), ctypes.byref(self.opts), ctypes.byref(h))
-        if rc != abi.OK:
-            raise NHError(rc, "orctm_nh_create")
-        self.h = h
+        if _handle is None:
+            h = ctypes.c_void_p()
+            rc = self.lib.orctm_nh_create(ctypes.byref(grid.c_struct()), ctypes.byref(self.opts), ctypes.byref(h))
+            if rc != abi.OK:
+                raise NHError(rc, "orctm_nh_create")
+            _handle = h
+        self.h = _handle
         self.shape = (grid.IE, grid.JE, grid.KE)
         self.fp32 = self.opts.precision == abi.MIXED
+
+    def clone(self) -> "NHSolver":
+        """A solver on the same grid sharing this one's assembled operator (orctm_nh_clone)."""
+        h = ctypes.c_void_p()
+        self._rc(self.lib.orctm_nh_clone(self.h, ctypes.byref(h)), "orctm_nh_clone")
+        return NHSolver(self.grid, self.opts, _handle=h)
+
+    def perturb_rhs(self, amplitude: float, seed: int) -> None:
+        self._rc(self.lib.orctm_nh_perturb_rhs(self.h, amplitude, seed), "orctm_nh_perturb_rhs")
 
     # -- the reference's three calls -------------------------------------------------------------
     def solve(self, divg: np.ndarray, pnh: np.ndarray, check: bool = True) -> dict:
