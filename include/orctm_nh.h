@@ -180,6 +180,12 @@ typedef struct orctm_nh_profile {
 } orctm_nh_profile_t;
 int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* prof, int32_t reset);
 
+/* Streaming bandwidth of the device: a plain 16-B-per-lane copy of `bytes` repeated `reps` times,
+ * timed with HIP events; *gbps = bytes read + written per second / 1e9. The copy is also the
+ * known-size reference for converting HBM byte counters. Allocates two buffers of `bytes`.
+ * No counterpart in the reference. */
+int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
+
 /* ------------------------------------------------------------------------------------------
  * Building blocks of the inner solve, exposed for verification. No counterpart in the reference
  * (its Krylov internals live inside PETSc). "inner" is the scaled operator C = S (-A),

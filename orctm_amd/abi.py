@@ -9,7 +9,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liborctm_nh.so")
+LIB_PATH = os.environ.get("ORCTM_NH_LIB", os.path.join(_HERE, "liborctm_nh.so"))
 
 OK, NOT_CONVERGED, BREAKDOWN, ERR_NONFINITE, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE = range(8)
 STATUS_NAMES = ["OK", "NOT_CONVERGED", "BREAKDOWN", "ERR_NONFINITE", "ERR_INVALID", "ERR_NOMEM", "ERR_HIP",
@@ -68,7 +68,7 @@ EXPORTS = [
     "orctm_nh_solve_resident", "orctm_nh_get_solution", "orctm_nh_apply", "orctm_nh_get_operator",
     "orctm_nh_get_dims", "orctm_nh_device_bytes", "orctm_nh_inner_apply", "orctm_nh_inner_precond",
     "orctm_nh_batch_create", "orctm_nh_batch_solve", "orctm_nh_batch_destroy", "orctm_nh_profile_read",
-    "orctm_nh_batch_profile_read", "orctm_nh_clone", "orctm_nh_perturb_rhs",
+    "orctm_nh_batch_profile_read", "orctm_nh_clone", "orctm_nh_perturb_rhs", "orctm_nh_copy_probe",
 ]
 
 _lib = None
@@ -109,6 +109,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.orctm_nh_batch_solve.argtypes = [vp, ctypes.POINTER(StatsC)]
     lib.orctm_nh_batch_destroy.argtypes = [vp]
     lib.orctm_nh_clone.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.orctm_nh_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     lib.orctm_nh_perturb_rhs.argtypes = [vp, ctypes.c_double, ctypes.c_uint64]
     lib.orctm_nh_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
     lib.orctm_nh_batch_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]
@@ -118,6 +119,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
             f.restype = ip
     _lib = lib
     return lib
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Achievable streaming bandwidth (GB/s) from a plain copy kernel on the device."""
+    g = ctypes.c_double()
+    rc = load().orctm_nh_copy_probe(device, nbytes, reps, ctypes.byref(g))
+    if rc != OK:
+        raise RuntimeError(f"orctm_nh_copy_probe: {status_name(rc)} {last_error()}")
+    return g.value
 
 
 def last_error() -> str:

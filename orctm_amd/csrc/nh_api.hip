@@ -855,6 +855,35 @@ int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed) {
   return map_err(e);
 }
 
+int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps) {
+  if (bytes < 16 || reps <= 0 || !gbps) return ORCTM_NH_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  hipError_t e = hipMalloc(&a, (size_t)bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, (size_t)bytes);
+  if (e == hipSuccess) e = hipMemset(a, 0, (size_t)bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy4(st, a, b, (size_t)bytes);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy4(st, a, b, (size_t)bytes);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *gbps = 2.0 * (double)(bytes / 16 * 16) * reps / (ms * 1e-3) / 1e9;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  hipFree(a);
+  hipFree(b);
+  return map_err(e);
+}
+
 int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_t** batch) {
   if (!batch || n <= 0 || !systems) return ORCTM_NH_ERR_INVALID;
   *batch = nullptr;

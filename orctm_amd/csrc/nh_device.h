@@ -13,7 +13,11 @@
 namespace onh {
 
 constexpr int TX = 64;           // tile width in i: one wavefront, consecutive columns
-constexpr int TY = 4;            // tile height in j: four wavefronts
+#ifndef ORCTM_TILE_Y
+#define ORCTM_TILE_Y 12
+#endif
+constexpr int TY = ORCTM_TILE_Y; // tile height in j: one wavefront per row
+static_assert(TY >= 4 && TY <= 16, "the matvec halo needs waves 0-2 (TY >= 4); blocks hold <= 1024 threads");
 constexpr int NT = TX * TY;      // threads per block
 constexpr int NCOEF = 7;
 

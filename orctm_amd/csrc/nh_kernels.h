@@ -37,6 +37,8 @@ hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* 
                        int raw = 0);
 hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
 hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+// dst <- src, bytes rounded down to 16 (bandwidth probe).
+hipError_t launch_copy4(hipStream_t st, const void* src, void* dst, size_t bytes);
 // y = A64 x on padded interior vectors (verification entry point).
 hipError_t launch_apply64(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* x,
                           double* y);

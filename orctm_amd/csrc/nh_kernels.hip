@@ -37,22 +37,28 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Sum of a and b over the block; valid in thread 0.
-template <typename T>
+// Sum of a and b over a block of NW wavefronts; valid in thread 0.
+template <typename T, int NW = NT / 64>
 __device__ __forceinline__ void block_sum2(T& a, T& b) {
-  __shared__ T sh[2 * (NT / 64)];
+  __shared__ T sh[2 * NW];
   a = wave_sum(a);
   b = wave_sum(b);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   __syncthreads();
   if (l == 0) {
     sh[w] = a;
-    sh[4 + w] = b;
+    sh[NW + w] = b;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    a = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-    b = ((sh[4] + sh[5]) + sh[6]) + sh[7];
+    T sa = sh[0], sb = sh[NW];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+      sa += sh[q];
+      sb += sh[NW + q];
+    }
+    a = sa;
+    b = sb;
   }
 }
 
@@ -222,8 +228,12 @@ __global__ __launch_bounds__(NT) void k_assemble(const Sys* __restrict__ systems
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = fmax(fmax(shm[0], shm[1]), fmax(shm[2], shm[3]));
-    int bflag = shb[0] | shb[1] | shb[2] | shb[3];
+    double m = shm[0];
+    int bflag = shb[0];
+    for (int q = 1; q < NT / 64; ++q) {
+      m = fmax(m, shm[q]);
+      bflag |= shb[q];
+    }
     if (bflag) atomicOr(bad, 1);
     if (isfinite(m)) atomicMax(smax_bits, (unsigned long long)__double_as_longlong(m));
   }
@@ -541,20 +551,25 @@ __global__ __launch_bounds__(NT) void k_fwd(const Sys* __restrict__ systems, con
   publish2(s, t.local, rho, rr);
 }
 
+// Threads of the matvec kernel: TY wavefronts own the tile rows (one column each, centre slots);
+// three more wavefronts own the halo: the north row, the south row and the 2*TY west/east cells.
+// Every thread runs the back-substitution recurrence of exactly one column, so no wavefront does
+// both centre and halo work between two barriers.
+constexpr int NW_BWD = TY + 3;
+constexpr int NT_BWD = 64 * NW_BWD;
+
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                            int raw) {
+__global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                int raw) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
   if (in->done != RUN) return;
-  const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x >> 6;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int i0 = t.bx * TX, j0 = t.by * TY;
-  const int i = i0 + tx, j = j0 + ty;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const bool active = i < nx && j < ny;
   const size_t P = s.plane, IP = s.pitch, NP = P * (size_t)nz;
   const bool first = in->first != 0;
   const int pb = in->pbuf;
@@ -568,98 +583,88 @@ __global__ __launch_bounds__(NT) void k_bwd(const Sys* __restrict__ systems, con
 
   __shared__ T lds[3][TY + 2][TX + 2];
 
-  int ci = i, cj = j;
-  const bool cval = map_ij(ci, cj, nx, ny, s.cyc_x, s.cyc_y);
-  const size_t coff = cval ? (size_t)cj * IP + ci : 0;
-  int hx = -1, hy = 0, hi = 0, hj = 0;
-  if (ty == 0) {
-    hi = i;
-    hj = j0 - 1;
-    hx = tx + 1;
-    hy = 0;
-  } else if (ty == 1) {
-    hi = i;
-    hj = j0 + TY;
-    hx = tx + 1;
-    hy = TY + 1;
-  } else if (ty == 2 && tx < 2 * TY) {
-    if (tx < TY) {
-      hi = i0 - 1;
-      hj = j0 + tx;
-      hx = 0;
-      hy = tx + 1;
-    } else {
-      hi = i0 + TX;
-      hj = j0 + tx - TY;
-      hx = TX + 1;
-      hy = tx - TY + 1;
-    }
+  // this thread's slot in the LDS plane and the column it follows
+  int gi, gj, sx, sy;
+  bool slot = true;
+  const bool centre = ty < TY;
+  if (centre) {
+    gi = i0 + tx;
+    gj = j0 + ty;
+    sx = tx + 1;
+    sy = ty + 1;
+  } else if (ty == TY) {
+    gi = i0 + tx;
+    gj = j0 - 1;
+    sx = tx + 1;
+    sy = 0;
+  } else if (ty == TY + 1) {
+    gi = i0 + tx;
+    gj = j0 + TY;
+    sx = tx + 1;
+    sy = TY + 1;
+  } else {
+    slot = tx < 2 * TY;
+    const int e = tx < TY ? 0 : 1, r = tx < TY ? tx : tx - TY;
+    gi = e ? i0 + TX : i0 - 1;
+    gj = j0 + (slot ? r : 0);
+    sx = e ? TX + 1 : 0;
+    sy = (slot ? r : 0) + 1;
   }
-  const bool has_halo = hx >= 0;
-  const bool hval = has_halo && map_ij(hi, hj, nx, ny, s.cyc_x, s.cyc_y);
-  const size_t hoff = hval ? (size_t)hj * IP + hi : 0;
+  const bool active = centre && gi < nx && gj < ny;
+  int mi = gi, mj = gj;
+  const bool val = slot && map_ij(mi, mj, nx, ny, s.cyc_x, s.cyc_y);
+  const size_t off = val ? (size_t)mj * IP + mi : 0;
 
   // Software pipeline over levels (bottom-up) in two register sets used alternately: while level k
   // is finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its
   // coefficients), set `nxt` receives w/l/p_old of level k-2 and the coefficients of level k-1.
   struct Stage {
-    T w, l, p, wh, lh, ph;
+    T w, l, p;
     T c[NCOEF];
   };
-  T zc = T(0), lc = T(0), zh = T(0), lh = T(0);  // z and l of the level below (centre, halo)
+  T zc = T(0), lc = T(0);  // z and l of this thread's column at the level below
   auto load_a = [&](int m, Stage& st) {
-    const size_t om = (size_t)(m > 0 ? m : 0) * P;
-    st.w = w[om + coff];
-    st.l = L[om + coff];
-    st.p = pold[om + coff];
-    if (has_halo) {
-      st.wh = w[om + hoff];
-      st.lh = L[om + hoff];
-      st.ph = pold[om + hoff];
-    }
+    const size_t om = (size_t)(m > 0 ? m : 0) * P + off;
+    st.w = w[om];
+    st.l = L[om];
+    st.p = pold[om];
   };
   auto load_c = [&](int m, Stage& st) {
-    const size_t cm = (size_t)(m > 0 ? m : 0) * P + coff;
+    if (!centre) return;
+    const size_t cm = (size_t)(m > 0 ? m : 0) * P + off;
 #pragma unroll
     for (int c = 0; c < NCOEF; ++c) st.c[c] = C[c * NP + cm];
   };
-  // back substitution and search-direction update of level m from w, l, p_old (centre and halo)
-  auto level = [&](int m, T wc, T Lc, T pc, T wh, T Lh, T ph) -> T {
-    const int buf = m % 3;
-    const size_t om = (size_t)m * P;
-    const T zm = fmad(-lc, zc, wc);
-    const T pm = first ? zm : fmad(beta, pc, zm);
+  // back substitution and search-direction update of level m for this thread's column
+  auto level = [&](int m, const Stage& st) -> T {
+    const T zm = fmad(-lc, zc, st.w);
+    const T pm = first ? zm : fmad(beta, st.p, zm);
     zc = zm;
-    lc = raw ? T(0) : Lc;
-    const T pout = cval ? pm : T(0);
-    lds[buf][ty + 1][tx + 1] = pout;
-    if (active) pnew[om + coff] = pm;
-    if (has_halo) {
-      const T zmh = fmad(-lh, zh, wh);
-      const T pmh = first ? zmh : fmad(beta, ph, zmh);
-      zh = zmh;
-      lh = raw ? T(0) : Lh;
-      lds[buf][hy][hx] = hval ? pmh : T(0);
-    }
+    lc = raw ? T(0) : st.l;
+    const T pout = val ? pm : T(0);
+    if (slot) lds[m % 3][sy][sx] = pout;
+    if (active) pnew[(size_t)m * P + off] = pm;
     return pout;
   };
   T p0 = T(0), pp1 = T(0), dot = T(0);
   auto step = [&](int k, Stage& cur, Stage& nxt) {
     load_a(k - 2, nxt);
     load_c(k - 1, nxt);
-    const T pm1 = k > 0 ? level(k - 1, cur.w, cur.l, cur.p, cur.wh, cur.lh, cur.ph) : T(0);
+    const T pm1 = k > 0 ? level(k - 1, cur) : T(0);
     __syncthreads();
-    const int b0 = k % 3;
-    T acc = cur.c[0] * p0;
-    acc = fmad(cur.c[1], lds[b0][ty + 1][tx], acc);
-    acc = fmad(cur.c[2], lds[b0][ty + 1][tx + 2], acc);
-    acc = fmad(cur.c[3], lds[b0][ty][tx + 1], acc);
-    acc = fmad(cur.c[4], lds[b0][ty + 2][tx + 1], acc);
-    acc = fmad(cur.c[5], pm1, acc);
-    acc = fmad(cur.c[6], pp1, acc);
-    if (active) {
-      q[(size_t)k * P + coff] = acc;
-      dot = fmad(p0, acc, dot);
+    if (centre) {
+      const int b0 = k % 3;
+      T acc = cur.c[0] * p0;
+      acc = fmad(cur.c[1], lds[b0][sy][sx - 1], acc);
+      acc = fmad(cur.c[2], lds[b0][sy][sx + 1], acc);
+      acc = fmad(cur.c[3], lds[b0][sy - 1][sx], acc);
+      acc = fmad(cur.c[4], lds[b0][sy + 1][sx], acc);
+      acc = fmad(cur.c[5], pm1, acc);
+      acc = fmad(cur.c[6], pp1, acc);
+      if (active) {
+        q[(size_t)k * P + off] = acc;
+        dot = fmad(p0, acc, dot);
+      }
     }
     pp1 = p0;
     p0 = pm1;
@@ -668,7 +673,7 @@ __global__ __launch_bounds__(NT) void k_bwd(const Sys* __restrict__ systems, con
   load_a(nz - 1, s1);
   load_a(nz - 2, s0);
   load_c(nz - 1, s0);
-  p0 = level(nz - 1, s1.w, s1.l, s1.p, s1.wh, s1.lh, s1.ph);
+  p0 = level(nz - 1, s1);
   __syncthreads();
   for (int k = nz - 1; k >= 0; k -= 2) {
     step(k, s0, s1);
@@ -676,7 +681,7 @@ __global__ __launch_bounds__(NT) void k_bwd(const Sys* __restrict__ systems, con
     step(k - 1, s1, s0);
   }
   T dummy = T(0);
-  block_sum2(dot, dummy);
+  block_sum2<T, NW_BWD>(dot, dummy);
   publish2(s, t.local, dot, dummy);
 }
 
@@ -783,6 +788,16 @@ __global__ __launch_bounds__(NT) void k_reduce(const Sys* __restrict__ systems) 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Plain copy (16 B per lane, grid-stride): the device's achievable streaming bandwidth, and the
+// known-size reference for converting HBM byte counters of the solver kernels.
+
+__global__ __launch_bounds__(256) void k_copy4(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n4; i += st) b[i] = a[i];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Launchers.
 
 #define LAUNCH(kern, grid, st, ...)                                \
@@ -866,8 +881,20 @@ hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
 
 hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
                        int raw) {
-  if (fp32) PHASE(k_bwd<float>, (launch_reduce<float, RED_BWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, raw);
-  PHASE(k_bwd<double>, (launch_reduce<double, RED_BWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, raw);
+  if (ntiles <= 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(k_bwd<float>, dim3(ntiles), dim3(NT_BWD), 0, st, d_sys, d_tiles, raw);
+  else
+    hipLaunchKernelGGL(k_bwd<double>, dim3(ntiles), dim3(NT_BWD), 0, st, d_sys, d_tiles, raw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return fp32 ? launch_reduce<float, RED_BWD>(st, d_sys, nsys) : launch_reduce<double, RED_BWD>(st, d_sys, nsys);
+}
+
+hipError_t launch_copy4(hipStream_t st, const void* src, void* dst, size_t bytes) {
+  const size_t n4 = bytes / 16;
+  hipLaunchKernelGGL(k_copy4, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n4);
+  return hipGetLastError();
 }
 
 hipError_t launch_apply64(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* x,
