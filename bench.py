@@ -30,6 +30,32 @@ from orctm_amd import abi, synth  # noqa: E402
 from orctm_amd.nh import NHSolver, options  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes of one launch, per cell and per (i, j) column, for word size W (4 FP32, 8 FP64).
+# k_bwd (back substitution + p update + 7-point matvec + p.q): reads w, l, p_old, ZW, ZP (5 words)
+#   and the uint16 coupling flags per cell, writes p and q (2 words); per column it reads the 7
+#   metric products and the bottom-face ZW (8 words).
+# k_fwd (x/r update + forward elimination + r.z, r.r): reads x, p, r, q, l, invD, writes x, r, w.
+def bwd_bytes(cells, columns, W):
+    return cells * (7 * W + 2) + columns * 8 * W
+
+
+def fwd_bytes(cells, W):
+    return cells * 9 * W
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel on this workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary kept in profiles/<name>.txt (line "traffic_bytes_per_launch: N"), or None if absent."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(float(ln.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 METRIC = "Poisson solves/s + HBM GB/s, mixed vs FP64 GPU at rel.res 1e-10; p50/p99 lat"
 
 
@@ -82,7 +108,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="regional", choices=["regional", "batch", "transect"])
+    ap.add_argument("--workload", default="regional", choices=["regional", "batch", "transect", "serve"])
+    ap.add_argument("--requests", type=int, default=24, help="serve workload: number of arrivals served")
+    ap.add_argument("--pool", type=int, default=6, help="serve workload: grid sizes pre-assembled")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve workload: Poisson arrival rate (1/s)")
     ap.add_argument("--nsys", type=int, default=64, help="batch workload: number of systems")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
@@ -96,6 +125,8 @@ def main():
         return bench_batch(args)
     if args.workload == "transect":
         return bench_transect(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     spec = synth.config2()
     if args.scale != 1.0:
         spec.nx = max(8, int(spec.nx * args.scale))
@@ -125,13 +156,12 @@ def main():
     inner = statistics.mean(st["inner_iters"] for st in mx["stats"])
     inner64 = statistics.mean(st["inner_iters"] for st in f64["stats"])
     cells = g.cells
-    # roofline of the matvec kernel k_bwd (back substitution + p update + 7-point matvec + p.q), FP32:
-    # reads w, l, p_old and 7 coefficients, writes p and q = 12 words = 48 B per cell.
-    # k_fwd (x/r update + forward elimination + r.z, r.r): reads x p r q l invD, writes x r w = 36 B.
+    columns = spec.nx * spec.ny
     prof = mx["prof"]
     mv_avg_ms = prof["matvec_ms"] / max(prof["matvec_launches"], 1)
     up_avg_ms = prof["update_ms"] / max(prof["update_launches"], 1)
-    achieved = 48.0 * cells / (mv_avg_ms * 1e-3) / 1e9
+    mv_bytes = bwd_bytes(cells, columns, 4)
+    achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
     prof64 = f64["prof"]
     mv64_ms = prof64["matvec_ms"] / max(prof64["matvec_launches"], 1)
     up64_ms = prof64["update_ms"] / max(prof64["update_launches"], 1)
@@ -168,17 +198,18 @@ def main():
         "latency_ms_p99_fp64": round(pct(f64["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (back substitution + p update + 7-point matvec + p.q)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": 48, "avg_launch_ms": round(mv_avg_ms, 5),
-                     "launches": prof["matvec_launches"]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
+                     "bytes_per_launch": mv_bytes, "bytes_per_cell": 30, "bytes_per_column": 32,
+                     "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
         "other_kernels": {
             "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 36,
-                             "GBps": round(36.0 * cells / (up_avg_ms * 1e-3) / 1e9, 1)},
-            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 96,
-                              "GBps": round(96.0 * cells / (mv64_ms * 1e-3) / 1e9, 1)},
+                             "GBps": round(fwd_bytes(cells, 4) / (up_avg_ms * 1e-3) / 1e9, 1)},
+            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 58, "bytes_per_column": 64,
+                              "GBps": round(bwd_bytes(cells, columns, 8) / (mv64_ms * 1e-3) / 1e9, 1)},
             "k_fwd<double>": {"avg_launch_ms": round(up64_ms, 5), "bytes_per_cell": 72,
-                              "GBps": round(72.0 * cells / (up64_ms * 1e-3) / 1e9, 1)},
+                              "GBps": round(fwd_bytes(cells, 8) / (up64_ms * 1e-3) / 1e9, 1)},
         },
+        "copy_probe_GBps": round(abi.copy_probe(1 << 30, 20), 1),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -230,6 +261,7 @@ def bench_batch(args):
         inner = sum(x["inner_iters"] for x in st)
         # cell-iterations actually processed per solve (converged systems drop out of later launches)
         cell_iters = sum(x["inner_iters"] * g.cells for x, (g, _) in zip(st, systems))
+        mv_bytes = sum(x["inner_iters"] * bwd_bytes(g.cells, g.nx * g.ny, 4) for x, (g, _) in zip(st, systems))
         # one-by-one reference timing of the same systems (sequential, same kernels)
         t1 = time.perf_counter()
         for s in solvers:
@@ -237,7 +269,7 @@ def bench_batch(args):
             if r["rc"] != abi.OK:
                 raise RuntimeError(f"single solve failed: {r}")
         seq = time.perf_counter() - t1
-        out[name] = dict(wall=wall, lat=lat, inner=inner, cell_iters=cell_iters,
+        out[name] = dict(wall=wall, lat=lat, inner=inner, cell_iters=cell_iters, mv_bytes=mv_bytes,
                          max_inner=max(x["inner_iters"] for x in st),
                          rel=max(x["rel_res"] for x in st), prof=prof, seq=seq)
         log(f"[bench] batch {name}: {len(systems) * args.steps / wall:.2f} solves/s, one-by-one "
@@ -249,7 +281,7 @@ def bench_batch(args):
     n = len(systems)
     prof = mx["prof"]
     # bytes moved by all matvec launches of the timed solves / their summed device time
-    mv_gbs = 48.0 * mx["cell_iters"] * args.steps / (prof["matvec_ms"] * 1e-3) / 1e9
+    mv_gbs = mx["mv_bytes"] * args.steps / (prof["matvec_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
@@ -265,7 +297,8 @@ def bench_batch(args):
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "48 B per active cell-iteration over the summed matvec launch time"},
+                     "note": "30 B per active cell-iteration + 32 B per active column-iteration over the summed "
+                             "matvec launch time"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -316,6 +349,77 @@ def bench_transect(args):
                                                    f"{st.inner_iters} iterations, {reps} solves"},
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals (rate 200/s), sizes log-uniform from 128x128x30 to 2048x1536x80,
+    10 time steps per request (RHS perturbed by 1 % N(0,1) per step, warm start from the resident
+    pressure), mixed precision to rel.res 1e-10, admission by free HBM, continuous batching.
+    The first `--requests` arrivals are served; sizes snap to `--pool` pre-assembled grids at
+    log-uniform quantiles (a request clones its grid's model: the operator is shared)."""
+    from orctm_amd.nh import NHServer
+
+    ts = [(q + 0.5) / args.pool for q in range(args.pool)]
+    models = []
+    t = time.perf_counter()
+    for q, tq in enumerate(ts):
+        nx, ny, nz = synth.config5_size(tq)
+        if args.scale != 1.0:
+            nx, ny = max(8, int(nx * args.scale)), max(8, int(ny * args.scale))
+        g, divg = synth.build(synth.GridSpec(nx, ny, nz, seed=synth.BASE_SEED + 500 + q))
+        m = NHSolver(g, options(precision=abi.MIXED, warm_start=True))
+        m.set_rhs(divg)
+        for a in ("dlxp", "dlyp", "dlxu", "dlyv", "ddpo", "weto"):
+            setattr(g, a, None)
+        del divg
+        models.append(m)
+        log(f"[bench] serve: model {q}: {nx}x{ny}x{nz} = {nx * ny * nz} cells")
+    log(f"[bench] serve: {len(models)} models assembled in {time.perf_counter() - t:.1f} s")
+    u = synth.uniform(synth.BASE_SEED, 505, 2 * args.requests)
+    gaps = -np.log(1.0 - u[: args.requests]) / args.rate
+    arrivals = np.cumsum(gaps)
+    sizes_t = u[args.requests:]
+    pick = [int(min(args.pool - 1, np.floor(x * args.pool))) for x in sizes_t]
+    nsteps = 10
+    with NHServer() as srv:
+        tickets = []
+        t0 = time.perf_counter()
+        for k in range(args.requests):
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(models[pick[k]], nsteps, 0.01, 9000 + 100 * k))
+        stats = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        info = srv.info()
+    bad = [s for s in stats if s["status"] != "OK"]
+    if bad:
+        raise RuntimeError(f"serve: {len(bad)} requests failed: {bad[:3]}")
+    lat = [s["t_done_ms"] - s["t_submit_ms"] for s in stats]
+    svc = [(s["t_done_ms"] - s["t_admit_ms"]) / nsteps for s in stats]
+    cells = [models[p].grid.cells for p in pick]
+    solves = nsteps * args.requests
+    line = {
+        "metric": METRIC, "value": round(solves / wall, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": solves, "warmup": 0, "ms_per_step": round(1e3 * wall / solves, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "data": "synthetic (seeded generator, regional-tile recipe per grid)",
+        "config": {"workload": f"serving: Poisson arrivals {args.rate:g}/s, first {args.requests} requests x "
+                               f"{nsteps} warm-started steps, sizes log-uniform 128x128x30..2048x1536x80 on a "
+                               f"{args.pool}-grid pool, mixed to rel.res 1e-10, continuous batching",
+                   "pool_cells": [m.grid.cells for m in models]},
+        "requests": args.requests, "rounds": info["rounds"],
+        "latency_ms_p50": round(pct(lat, 50), 1), "latency_ms_p99": round(pct(lat, 99), 1),
+        "service_ms_per_step_p50": round(pct(svc, 50), 2), "service_ms_per_step_p99": round(pct(svc, 99), 2),
+        "cell_solves_per_s": round(nsteps * sum(cells) / wall, 1),
+        "mean_inner_iters_per_step": round(sum(s["inner_iters"] for s in stats) / solves, 2),
+        "max_rel_res": max(s["rel_res"] for s in stats),
+        "offered_load_note": "arrival rate far above service capacity: latency is queue-dominated",
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    for m in models:
+        m.close()
 
 
 if __name__ == "__main__":

@@ -41,7 +41,8 @@ enum orctm_nh_status {
   ORCTM_NH_ERR_INVALID = 4,     /* invalid geometry or arguments */
   ORCTM_NH_ERR_NOMEM = 5,       /* device allocation failed or admission refused */
   ORCTM_NH_ERR_HIP = 6,         /* HIP runtime error */
-  ORCTM_NH_ERR_NODEVICE = 7     /* no usable gfx950 device */
+  ORCTM_NH_ERR_NODEVICE = 7,    /* no usable gfx950 device */
+  ORCTM_NH_PENDING = 8          /* serving: the request has not finished yet */
 };
 
 /* Precision modes. */
@@ -91,7 +92,10 @@ typedef struct orctm_nh_options {
   int32_t check_every;  /* inner iterations per device-side chunk between host polls (default 16) */
   int32_t device;       /* HIP device ordinal (default 0) */
   int32_t profile;      /* 1: time the inner kernels with HIP events (orctm_nh_profile_read); default 0 */
-  int32_t reserved[5];
+  int32_t tile_rows;    /* rows (j) per tile of the inner-iteration kernels: 0 (default) = tuned per size
+                           class at create / batch create; 4, 8 or 12 = fixed. Results are the same bits
+                           for every choice (block sums are formed over fixed 4-row groups). */
+  int32_t reserved[4];
 } orctm_nh_options_t;
 
 /* Per-solve statistics. */
@@ -99,7 +103,7 @@ typedef struct orctm_nh_stats {
   int32_t status;       /* enum orctm_nh_status of this solve */
   int32_t outer_iters;  /* FP64 refinement steps taken */
   int32_t inner_iters;  /* total inner PCG iterations */
-  int32_t reserved0;
+  int32_t tile_rows;    /* tile height the inner-iteration kernels ran with for this system */
   double rel_res;       /* final ||b - A p||_2 / ||b||_2 in FP64 with the reference operator */
   double bnorm;         /* ||b||_2 */
   double solve_ms;      /* device time of the solve (HIP events), excluding host<->device copies */
@@ -155,6 +159,9 @@ int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz);
 /* Bytes of device memory the handle holds (its own state plus the operator it uses). */
 int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes);
 
+/* Bytes of device memory this handle owns alone (its state, not the operator it may share). */
+int orctm_nh_state_bytes(orctm_nh_t* h, int64_t* bytes);
+
 /* A new handle on the same grid that shares src's assembled operator (read only, freed with the last
  * handle using it) and owns its own right-hand side (a copy of src's), solution (zero) and work
  * vectors: about 40 B/cell (MIXED) instead of ~140. For many states on one grid: ensemble members,
@@ -207,6 +214,53 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
 int orctm_nh_batch_solve(orctm_nh_batch_t* batch, orctm_nh_stats_t* stats);
 int orctm_nh_batch_destroy(orctm_nh_batch_t* batch);
 int orctm_nh_batch_profile_read(orctm_nh_batch_t* batch, orctm_nh_profile_t* prof, int32_t reset);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching. No counterpart in the reference (ORCTM solves
+ * once per model time step on its own ranks). A request is a grid -- given as a "model" handle whose
+ * operator and resident right-hand side (orctm_nh_set_rhs) define it -- and nsteps consecutive time
+ * steps: between steps the right-hand side is perturbed on the device (orctm_nh_perturb_rhs,
+ * amplitude `perturb`, stream seed + step) and the solve warm-starts from the pressure kept on the
+ * device. The server thread admits queued requests (FIFO) while the slot count and the device-memory
+ * budget allow (each admitted request holds a clone of its model: the operator is shared), advances
+ * all admitted requests by one step per batched solve, and admits new arrivals between rounds.
+ * A request larger than the whole budget is refused with ORCTM_NH_ERR_NOMEM. The model handle must
+ * stay alive and unchanged until its requests have been admitted. */
+typedef struct orctm_nh_server orctm_nh_server_t;
+
+typedef struct orctm_nh_server_options {
+  int64_t hbm_budget;   /* bytes of device memory for admitted requests' state (<= 0: 90 % of free) */
+  int32_t max_active;   /* admitted requests at most (default 64) */
+  int32_t device;       /* HIP device ordinal (default 0) */
+  int32_t reserved[8];
+} orctm_nh_server_options_t;
+
+typedef struct orctm_nh_request_stats {
+  int32_t status;       /* first non-OK step status, else ORCTM_NH_OK */
+  int32_t steps_done;
+  int32_t inner_iters;  /* summed over the steps */
+  int32_t outer_iters;
+  double rel_res;       /* of the last step */
+  double t_submit_ms;   /* server clock: milliseconds since orctm_nh_server_create */
+  double t_admit_ms;
+  double t_done_ms;
+  double reserved[4];
+} orctm_nh_request_stats_t;
+
+void orctm_nh_server_default_options(orctm_nh_server_options_t* opts);
+int orctm_nh_server_create(const orctm_nh_server_options_t* opts, orctm_nh_server_t** server);
+/* Queues a request; pnh_out (may be NULL): Fortran (IE,JE,KE) array that receives the interior of
+ * the last step's solution before the request completes. */
+int orctm_nh_server_submit(orctm_nh_server_t* server, orctm_nh_t* model, int32_t nsteps, double perturb,
+                           uint64_t seed, double* pnh_out, int64_t* ticket);
+/* ORCTM_NH_OK when finished (stats filled), ORCTM_NH_PENDING otherwise. */
+int orctm_nh_server_poll(orctm_nh_server_t* server, int64_t ticket, orctm_nh_request_stats_t* stats);
+int orctm_nh_server_wait(orctm_nh_server_t* server, int64_t ticket, orctm_nh_request_stats_t* stats);
+int orctm_nh_server_drain(orctm_nh_server_t* server);
+int orctm_nh_server_info(orctm_nh_server_t* server, int64_t* rounds, int64_t* bytes_in_use, int32_t* active,
+                         int32_t* queued);
+/* Finishes every queued and admitted request, then stops the server thread. */
+int orctm_nh_server_destroy(orctm_nh_server_t* server);
 
 #ifdef __cplusplus
 }

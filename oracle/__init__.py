@@ -52,7 +52,11 @@ def lib() -> ctypes.CDLL:
         L.orc_pcg32.argtypes = [i, i, i, i, i, _fp, _fp, _fp, _fp, _fp, _fp, ctypes.c_float, i,
                                 ctypes.POINTER(ctypes.c_int)]
         L.orc_solve.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp, i, ctypes.c_double, ctypes.c_double, i, i,
-                                ctypes.POINTER(StatsC)]
+                                ctypes.POINTER(StatsC), vp]
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        L.orc_compact.argtypes = [ctypes.POINTER(GridC), _dp, _dp, u16p, _dp]
+        L.orc_rebuild64.argtypes = [i, i, i, _dp, _dp, u16p, _dp, _dp]
+        L.orc_inner_coef32_compact.argtypes = [i, i, i, _dp, _dp, u16p, _dp, _fp]
         L.orc_fgmres.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp, i, ctypes.c_double, i, ctypes.POINTER(StatsC)]
         L.orc_num_threads.restype = i
         _lib = L
@@ -80,6 +84,36 @@ def assemble(grid):
     return A, S
 
 
+def compact(grid):
+    """(zw[(nz+1)*P], zp[nz*P], flags[nz*P] uint16, col[7*P]) -- the operator's compact form."""
+    P = grid.nx * grid.ny
+    zw = np.zeros((grid.nz + 1) * P)
+    zp = np.zeros(grid.nz * P)
+    fl = np.zeros(grid.nz * P, dtype=np.uint16)
+    col = np.zeros(7 * P)
+    rc = lib().orc_compact(ctypes.byref(grid.c_struct()), _d(zw), _d(zp),
+                           fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _d(col))
+    if rc != 0:
+        raise ValueError(f"orc_compact status {rc}")
+    return zw, zp, fl, col
+
+
+def rebuild64(grid, zw, zp, fl, col):
+    A = np.zeros((NC, grid.cells))
+    lib().orc_rebuild64(grid.nx, grid.ny, grid.nz, _d(zw), _d(zp), fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                        _d(col), _d(A))
+    return A
+
+
+def inner_coef32(grid):
+    """FP32 inner operator S*(-A) evaluated in float from the compact form (the device's arithmetic)."""
+    zw, zp, fl, col = compact(grid)
+    C = np.zeros((NC, grid.cells), dtype=np.float32)
+    lib().orc_inner_coef32_compact(grid.nx, grid.ny, grid.nz, _d(zw), _d(zp),
+                                   fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _d(col), _f(C))
+    return C
+
+
 def rhs(grid, divg):
     """b = DIVG*WETO on the interior (mo_petsc.f90:376), natural order."""
     b = np.zeros(grid.cells)
@@ -104,8 +138,12 @@ def matvec(grid, A, x):
     return y
 
 
-def inner_coef(A, S, dtype=np.float64):
+def inner_coef(A, S, dtype=np.float64, grid=None):
+    """Inner operator C = S*(-A): FP64 from the assembled A and S; FP32 from the compact form of
+    `grid` in float arithmetic (as the device evaluates it), or from A and S if no grid is given."""
     N = S.size
+    if dtype == np.float32 and grid is not None:
+        return inner_coef32(grid)
     if dtype == np.float32:
         C = np.zeros((NC, N), dtype=np.float32)
         lib().orc_inner_coef32(N, _d(np.ascontiguousarray(A)), _d(S), _f(C))
@@ -144,9 +182,10 @@ def solve(grid, A, S, b, precision=0, rtol=1e-10, rtol_inner=None, maxit=10000, 
         rtol_inner = 1e-5 if precision == 1 else 0.0
     p = np.zeros(grid.cells) if x0 is None else np.array(x0, dtype=np.float64)
     st = StatsC()
+    Cin = inner_coef32(grid) if precision == 1 else None
     lib().orc_solve(grid.nx, grid.ny, grid.nz, *cyc(grid), _d(np.ascontiguousarray(A)), _d(S),
                     _d(np.ascontiguousarray(b)), _d(p), precision, rtol, rtol_inner, maxit, max_outer,
-                    ctypes.byref(st))
+                    ctypes.byref(st), Cin.ctypes.data if Cin is not None else None)
     return p, st
 
 

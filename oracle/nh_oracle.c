@@ -12,6 +12,9 @@
  *   orc_rhs           ComputeRHS, mo_petsc.f90:298-376 (active statement :376, b = DIVG*WETO)
  *   orc_matvec64      the AIJ MatMult KSPSolve applies to that matrix (PETSc internal); here as a
  *                     fixed-order fma chain centre,W,E,N,S,U,D which the device kernel also uses
+ *   orc_compact       the compact form of the same operator (per-cell ZW/ZP + flag bits + per-column
+ *                     metric products) the device keeps; orc_rebuild64 rebuilds compute_matrix's values
+ *                     from it bit for bit, orc_inner_coef32_compact evaluates the FP32 inner operator
  *   orc_inner_*       the scaled operator C = S*(-A), S = DLXP*DLYP/ZP, in FP64 or FP32, its
  *                     vertical-line preconditioner and the PCG/refinement driver. These have no
  *                     counterpart in the reference (which runs FGMRES + PCMG inside PETSc,
@@ -176,6 +179,129 @@ int orc_assemble(const orctm_nh_grid_t* g, double* A, double* S) {
   return bad ? ORCTM_NH_ERR_INVALID : ORCTM_NH_OK;
 }
 
+/* ------------------------------------------------------------------------------------------- */
+/* Compact form of the same operator: everything compute_matrix writes is fixed by per-cell ZW/ZP,
+   12 flag bits and 7 per-column metric products. The device keeps only this form and rebuilds each
+   row on the fly; orc_row64 rebuilds it here with the operations orc_assemble uses.
+     flags bit 0     WETO(I,J,K) == 1
+           bits 1-4  coupling present W, E, N, S   (mo_petsc.f90:692-764)
+           bit 5     coupling present U (K != 1)
+           bit 6     coupling present D (K != KE and WETO(K+1) == 1)
+           bits 7-8  X_C: 0 (XUw+XUe)*XP, 1 XUe*XP, 2 XUw*XP, 3 zero  (mo_petsc.f90:630-640)
+           bits 9-10 Y_C: 0 (YVn+YVs)*YP, 1 YVs*YP, 2 YVn*YP, 3 zero  (mo_petsc.f90:643-653)
+           bit 11    Z_C: 0 (ZW(K)+ZW(K+1))*ZP(K), 1 ZW(K)*ZP(K)      (mo_petsc.f90:657-665)
+     col planes 0-6  XUw*XP, XUe*XP, YVn*YP, YVs*YP, (XUw+XUe)*XP, (YVn+YVs)*YP, DLXP*DLYP        */
+
+int orc_compact(const orctm_nh_grid_t* g, double* zw, double* zp, uint16_t* flags, double* col) {
+  if (!grid_ok(g) || !zw || !zp || !flags || !col) return ORCTM_NH_ERR_INVALID;
+  const int IE = g->ie, JE = g->je, KE = g->ke;
+  const int IE1 = IE - 1, JE1 = JE - 1;
+  const int nx = IE - 2, ny = JE - 2;
+  const size_t P = (size_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int J = 2; J <= JE1; ++J) {
+    for (int I = 2; I <= IE1; ++I) {
+      const size_t c = (size_t)(I - 2) + (size_t)nx * (J - 2);
+      double* zwc = (double*)malloc(sizeof(double) * (2 * (size_t)KE + 1));
+      double* zpc = zwc + KE + 1;
+      column_zw_zp(g, I, J, zwc, zpc);
+      const double xuw = 1.0 / U2(g->dlxu, I - 1, J), xue = 1.0 / U2(g->dlxu, I, J), xp = 1.0 / P2(g->dlxp, I, J);
+      const double yvn = 1.0 / V2(g->dlyv, I, J - 1), yvs = 1.0 / V2(g->dlyv, I, J), yp = 1.0 / P2(g->dlyp, I, J);
+      col[0 * P + c] = xuw * xp;
+      col[1 * P + c] = xue * xp;
+      col[2 * P + c] = yvn * yp;
+      col[3 * P + c] = yvs * yp;
+      col[4 * P + c] = (xuw + xue) * xp;
+      col[5 * P + c] = (yvn + yvs) * yp;
+      col[6 * P + c] = P2(g->dlxp, I, J) * P2(g->dlyp, I, J);
+      for (int K = 1; K <= KE + 1; ++K) zw[(size_t)(K - 1) * P + c] = zwc[K - 1];
+      for (int K = 1; K <= KE; ++K) {
+        zp[(size_t)(K - 1) * P + c] = zpc[K - 1];
+        unsigned f = 0;
+        if (P3(g->weto, I, J, K) != 0.0) {
+          const double ww = P3(g->weto, I - 1, J, K), we = P3(g->weto, I + 1, J, K);
+          const double wn = P3(g->weto, I, J - 1, K), ws = P3(g->weto, I, J + 1, K);
+          f |= 1u;
+          if (ww == 1.0 && (g->cyclic_x || I != 2)) f |= 1u << 1;
+          if (we == 1.0 && (g->cyclic_x || I != IE1)) f |= 1u << 2;
+          if (wn == 1.0 && (g->cyclic_y || J != 2)) f |= 1u << 3;
+          if (ws == 1.0 && (g->cyclic_y || J != JE1)) f |= 1u << 4;
+          if (K != 1) f |= 1u << 5;
+          if (K != KE && P3(g->weto, I, J, K + 1) == 1.0) f |= 1u << 6;
+          unsigned xc = 0, yc = 0, zc = 0;
+          if (ww == 0.0 && we == 1.0) xc = 1;
+          else if (ww == 1.0 && we == 0.0) xc = 2;
+          else if (ww == 0.0 && we == 0.0) xc = 3;
+          if (wn == 0.0 && ws == 1.0) yc = 1;
+          else if (wn == 1.0 && ws == 0.0) yc = 2;
+          else if (wn == 0.0 && ws == 0.0) yc = 3;
+          if (K == 1) zc = 0;
+          else if (K == KE || P3(g->weto, I, J, K + 1) == 0.0) zc = 1;
+          f |= xc << 7 | yc << 9 | zc << 11;
+        }
+        flags[(size_t)(K - 1) * P + c] = (uint16_t)f;
+      }
+      free(zwc);
+    }
+  }
+  return ORCTM_NH_OK;
+}
+
+/* One row of the operator from the compact form, in REAL arithmetic (double: bit-exact with
+   orc_assemble; float: the inner operator's arithmetic). c7 receives centre, W, E, N, S, U, D. */
+#define ORC_ROW(REAL, NAME)                                                                                 \
+  static void NAME(const REAL* colc, REAL zwk, REAL zwk1, REAL zpk, unsigned f, REAL* c7) {                 \
+    const REAL xw = colc[0], xe = colc[1], yn = colc[2], ys = colc[3];                                      \
+    const unsigned xc = (f >> 7) & 3u, yc = (f >> 9) & 3u;                                                  \
+    const REAL X_C = xc == 0 ? colc[4] : xc == 1 ? xe : xc == 2 ? xw : (REAL)0;                             \
+    const REAL Y_C = yc == 0 ? colc[5] : yc == 1 ? ys : yc == 2 ? yn : (REAL)0;                             \
+    const REAL Z_C = ((f >> 11) & 1u) ? zwk * zpk : (zwk + zwk1) * zpk;                                     \
+    c7[0] = -(X_C + Y_C + Z_C);                                                                             \
+    c7[1] = (f & (1u << 1)) ? xw : (REAL)0;                                                                 \
+    c7[2] = (f & (1u << 2)) ? xe : (REAL)0;                                                                 \
+    c7[3] = (f & (1u << 3)) ? yn : (REAL)0;                                                                 \
+    c7[4] = (f & (1u << 4)) ? ys : (REAL)0;                                                                 \
+    c7[5] = (f & (1u << 5)) ? zwk * zpk : (REAL)0;                                                          \
+    c7[6] = (f & (1u << 6)) ? zwk1 * zpk : (REAL)0;                                                         \
+  }
+ORC_ROW(double, row64)
+ORC_ROW(float, row32)
+
+/* The reference operator rebuilt from the compact form (must equal orc_assemble bit for bit). */
+void orc_rebuild64(int nx, int ny, int nz, const double* zw, const double* zp, const uint16_t* flags,
+                   const double* col, double* A) {
+  const size_t P = (size_t)nx * ny, N = P * nz;
+#pragma omp parallel for schedule(static)
+  for (size_t c = 0; c < P; ++c) {
+    double cc[7], c7[7];
+    for (int q = 0; q < 7; ++q) cc[q] = col[(size_t)q * P + c];
+    for (int k = 0; k < nz; ++k) {
+      const size_t n = c + P * k;
+      row64(cc, zw[n], zw[n + P], zp[n], flags[n], c7);
+      for (int q = 0; q < 7; ++q) A[(size_t)q * N + n] = c7[q];
+    }
+  }
+}
+
+/* The FP32 inner operator C = S*(-A), S = DLXP*DLYP/ZP, evaluated in float from the compact form
+   rounded to float (what the device matvec computes on the fly). */
+void orc_inner_coef32_compact(int nx, int ny, int nz, const double* zw, const double* zp, const uint16_t* flags,
+                              const double* col, float* C) {
+  const size_t P = (size_t)nx * ny, N = P * nz;
+#pragma omp parallel for schedule(static)
+  for (size_t c = 0; c < P; ++c) {
+    float cc[7], c7[7];
+    for (int q = 0; q < 7; ++q) cc[q] = (float)col[(size_t)q * P + c];
+    for (int k = 0; k < nz; ++k) {
+      const size_t n = c + P * k;
+      const float zpk = (float)zp[n];
+      row32(cc, (float)zw[n], (float)zw[n + P], zpk, flags[n], c7);
+      const float S = cc[6] / zpk;
+      for (int q = 0; q < 7; ++q) C[(size_t)q * N + n] = S * (-c7[q]);
+    }
+  }
+}
+
 /* ComputeRHS, mo_petsc.f90:298-376: b = DIVG*WETO on the interior. */
 int orc_rhs(const orctm_nh_grid_t* g, const double* divg, double* b) {
   if (!grid_ok(g) || !divg || !b) return ORCTM_NH_ERR_INVALID;
@@ -248,7 +374,7 @@ static double dot64(size_t n, const double* a, const double* b) {
  */
 int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, const double* S, const double* b,
               double* p, int precision, double rtol, double rtol_inner, int maxit, int max_outer,
-              orctm_nh_stats_t* st) {
+              orctm_nh_stats_t* st, const void* Cin) {
   const size_t N = (size_t)nx * ny * nz;
   orctm_nh_stats_t s;
   memset(&s, 0, sizeof s);
@@ -279,11 +405,12 @@ int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, con
   rin = malloc(es * N);
   x = malloc(es * N);
   work = malloc(es * 4 * N);
+  if (Cin) memcpy(C, Cin, es * NC * N);
   if (precision == ORCTM_NH_FP64) {
-    orc_inner_coef64(N, A, S, (double*)C);
+    if (!Cin) orc_inner_coef64(N, A, S, (double*)C);
     orc_line_factor64(nx, ny, nz, (const double*)C, (double*)f1, (double*)f2);
   } else {
-    orc_inner_coef32(N, A, S, (float*)C);
+    if (!Cin) orc_inner_coef32(N, A, S, (float*)C);
     orc_line_factor32(nx, ny, nz, (const float*)C, (float*)f1, (float*)f2);
   }
   /* the inner right-hand side is normalised: rin = S.*(-r) * gamma, gamma = 1/(max S * ||r_prev||),

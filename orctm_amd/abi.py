@@ -11,9 +11,9 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ORCTM_NH_LIB", os.path.join(_HERE, "liborctm_nh.so"))
 
-OK, NOT_CONVERGED, BREAKDOWN, ERR_NONFINITE, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE = range(8)
+OK, NOT_CONVERGED, BREAKDOWN, ERR_NONFINITE, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE, PENDING = range(9)
 STATUS_NAMES = ["OK", "NOT_CONVERGED", "BREAKDOWN", "ERR_NONFINITE", "ERR_INVALID", "ERR_NOMEM", "ERR_HIP",
-                "ERR_NODEVICE"]
+                "ERR_NODEVICE", "PENDING"]
 FP64, MIXED = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -35,7 +35,7 @@ class OptionsC(ctypes.Structure):
         ("precision", ctypes.c_int32), ("maxit", ctypes.c_int32), ("max_outer", ctypes.c_int32),
         ("warm_start", ctypes.c_int32), ("rtol", ctypes.c_double), ("rtol_inner", ctypes.c_double),
         ("check_every", ctypes.c_int32), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("tile_rows", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -50,14 +50,34 @@ class ProfileC(ctypes.Structure):
 class StatsC(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int32), ("outer_iters", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32), ("rel_res", ctypes.c_double), ("bnorm", ctypes.c_double),
+        ("tile_rows", ctypes.c_int32), ("rel_res", ctypes.c_double), ("bnorm", ctypes.c_double),
         ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("reserved", ctypes.c_double * 4),
     ]
 
     def as_dict(self):
         return {"status": STATUS_NAMES[self.status] if 0 <= self.status < len(STATUS_NAMES) else self.status,
                 "outer_iters": self.outer_iters, "inner_iters": self.inner_iters, "rel_res": self.rel_res,
-                "bnorm": self.bnorm, "solve_ms": self.solve_ms, "total_ms": self.total_ms}
+                "bnorm": self.bnorm, "solve_ms": self.solve_ms, "total_ms": self.total_ms,
+                "tile_rows": self.tile_rows}
+
+
+class ServerOptionsC(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("max_active", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 8)]
+
+
+class RequestStatsC(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("steps_done", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
+        ("outer_iters", ctypes.c_int32), ("rel_res", ctypes.c_double), ("t_submit_ms", ctypes.c_double),
+        ("t_admit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double), ("reserved", ctypes.c_double * 4),
+    ]
+
+    def as_dict(self):
+        return {"status": STATUS_NAMES[self.status] if 0 <= self.status < len(STATUS_NAMES) else self.status,
+                "steps_done": self.steps_done, "inner_iters": self.inner_iters, "outer_iters": self.outer_iters,
+                "rel_res": self.rel_res, "t_submit_ms": self.t_submit_ms, "t_admit_ms": self.t_admit_ms,
+                "t_done_ms": self.t_done_ms}
 
 
 # Every symbol include/orctm_nh.h declares (checked by the CPU test suite).
@@ -69,6 +89,9 @@ EXPORTS = [
     "orctm_nh_get_dims", "orctm_nh_device_bytes", "orctm_nh_inner_apply", "orctm_nh_inner_precond",
     "orctm_nh_batch_create", "orctm_nh_batch_solve", "orctm_nh_batch_destroy", "orctm_nh_profile_read",
     "orctm_nh_batch_profile_read", "orctm_nh_clone", "orctm_nh_perturb_rhs", "orctm_nh_copy_probe",
+    "orctm_nh_state_bytes", "orctm_nh_server_default_options", "orctm_nh_server_create", "orctm_nh_server_submit",
+    "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
+    "orctm_nh_server_destroy",
 ]
 
 _lib = None
@@ -109,6 +132,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.orctm_nh_batch_solve.argtypes = [vp, ctypes.POINTER(StatsC)]
     lib.orctm_nh_batch_destroy.argtypes = [vp]
     lib.orctm_nh_clone.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.orctm_nh_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.orctm_nh_server_default_options.argtypes = [ctypes.POINTER(ServerOptionsC)]
+    lib.orctm_nh_server_default_options.restype = None
+    lib.orctm_nh_server_create.argtypes = [ctypes.POINTER(ServerOptionsC), ctypes.POINTER(vp)]
+    lib.orctm_nh_server_submit.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, _dp,
+                                           ctypes.POINTER(ctypes.c_int64)]
+    lib.orctm_nh_server_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RequestStatsC)]
+    lib.orctm_nh_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RequestStatsC)]
+    lib.orctm_nh_server_drain.argtypes = [vp]
+    lib.orctm_nh_server_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    lib.orctm_nh_server_destroy.argtypes = [vp]
     lib.orctm_nh_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     lib.orctm_nh_perturb_rhs.argtypes = [vp, ctypes.c_double, ctypes.c_uint64]
     lib.orctm_nh_profile_read.argtypes = [vp, ctypes.POINTER(ProfileC), ctypes.c_int32]

@@ -12,9 +12,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "../../include/orctm_nh.h"
@@ -24,10 +26,17 @@ using namespace onh;
 
 namespace {
 
+// Tiles of the inner-iteration kernels for one tile height (systems grouped by their tuned height).
+struct Group {
+  int h = 12;
+  int ntiles = 0;
+  Tile* d_tiles = nullptr;
+};
+
 struct Ctx {
   int nsys = 0;
   bool fp32 = true;
-  int ntiles = 0;
+  int ntiles = 0;  // set-up tiling (64 x TY), d_tiles
   int max_outer = 0;
   int maxit = 10000;
   int check_every = 16;
@@ -39,11 +48,13 @@ struct Ctx {
   Outer* h_outer = nullptr;  // pinned
   void* h_inner = nullptr;   // pinned
   hipStream_t stream = nullptr;
+  std::vector<Group> groups;       // inner-iteration tilings, one per tile height in use
+  std::vector<int> heights;        // tile height of each system
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // optional kernel timing: events around every matvec / update launch inside the graph
+  // optional kernel timing: events around every matvec / update launch (direct launches)
   bool prof = false;
-  std::vector<hipEvent_t> kev;      // 3 per captured iteration
+  std::vector<hipEvent_t> kev;      // 4 per iteration of a chunk
   std::vector<int> iter_seen;       // per system, inner iterations already accounted
   double mv_ms = 0.0, up_ms = 0.0;
   long long mv_n = 0, up_n = 0, cells = 0;
@@ -63,8 +74,9 @@ int inner_done(const Ctx& c, int q) {
 
 }  // namespace
 
-// Device arrays of one assembled operator (A64, S, C, line factors, wet mask, max S). Shared, read
-// only, by every handle cloned from the one that assembled it.
+// Device arrays of one assembled operator (compact form: ZW, ZP, flag bits, per-column metric
+// products, their inner-precision copies, line-preconditioner factors, max S). Shared, read only,
+// by every handle cloned from the one that assembled it.
 struct OpData {
   int device = 0;
   std::vector<void*> allocs;
@@ -90,7 +102,6 @@ struct orctm_nh {
   double* d_nb = nullptr;          // ||b|| written by k_rhs
   unsigned long long* d_smax = nullptr;
   Ctx ctx;                         // single-system context
-  std::vector<Tile> tiles;         // this system's tiles (local order)
   std::mutex mu;
 };
 
@@ -168,7 +179,7 @@ int alloc_state(orctm_nh_t* h) {
   SALLOC(pa, NP * es, void*);
   SALLOC(pb, NP * es, void*);
   SALLOC(q, NP * es, void*);
-  SALLOC(partials, 2 * (size_t)s.ntiles * sizeof(double), double*);
+  SALLOC(partials, 2 * (size_t)s.nslots * sizeof(double), double*);
   SALLOC(nbp, sizeof(double), double*);
 #undef SALLOC
   h->d_nb = s.nbp;
@@ -189,10 +200,10 @@ void free_all(orctm_nh_t* h) {
   h->allocs.clear();
 }
 
-// Tiles of one system in natural (row-major) order.
-std::vector<Tile> system_tiles(int sys_index, int nx, int ny) {
+// Tiles (64 x h columns) of one system in natural (row-major) order.
+std::vector<Tile> system_tiles(int sys_index, int nx, int ny, int h) {
   std::vector<Tile> t;
-  const int tx = (nx + TX - 1) / TX, ty = (ny + TY - 1) / TY;
+  const int tx = (nx + TX - 1) / TX, ty = (ny + h - 1) / h;
   t.reserve((size_t)tx * ty);
   int local = 0;
   for (int by = 0; by < ty; ++by)
@@ -215,6 +226,8 @@ void ctx_destroy(Ctx& c) {
   if (c.graph) hipGraphExecDestroy(c.graph);
   if (c.d_sys) hipFree(c.d_sys);
   if (c.d_tiles) hipFree(c.d_tiles);
+  for (Group& g : c.groups)
+    if (g.d_tiles) hipFree(g.d_tiles);
   if (c.d_outer) hipFree(c.d_outer);
   if (c.d_inner) hipFree(c.d_inner);
   if (c.h_outer) hipHostFree(c.h_outer);
@@ -226,9 +239,157 @@ void ctx_destroy(Ctx& c) {
   c = Ctx{};
 }
 
+// One inner-iteration phase over every tile group, then the phase's per-system reduction. With e0/e1
+// given, events bracket the phase's kernels (not the reduction) for kernel timing.
+hipError_t iter_bwd(Ctx& c, int raw, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (e0) HIPCHK(hipEventRecord(e0, c.stream));
+  for (const Group& g : c.groups) HIPCHK(launch_bwd(c.stream, c.fp32, g.h, c.d_sys, g.d_tiles, g.ntiles, raw));
+  if (e1) HIPCHK(hipEventRecord(e1, c.stream));
+  return launch_inner_reduce(c.stream, c.fp32, PH_BWD, c.d_sys, c.nsys);
+}
+
+hipError_t iter_fwd(Ctx& c, bool init, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (e0) HIPCHK(hipEventRecord(e0, c.stream));
+  for (const Group& g : c.groups) HIPCHK(launch_fwd(c.stream, c.fp32, init, g.h, c.d_sys, g.d_tiles, g.ntiles));
+  if (e1) HIPCHK(hipEventRecord(e1, c.stream));
+  return launch_inner_reduce(c.stream, c.fp32, init ? PH_INIT : PH_FWD, c.d_sys, c.nsys);
+}
+
+// ---- tile-height tuning --------------------------------------------------------------------------
+// The inner kernels run on 64 x h tiles, h in {4, 8, 12}. Small h gives more blocks (small systems
+// fill the 256 CUs) and lighter blocks (two blocks per CU where registers are tight, FP64); large h
+// re-reads fewer halo rows in the matvec. The best h is measured per size class (ceil(log2) of the
+// column count nx*ny, the precision, and ceil(log2) of the number of systems of the class in the
+// context) and kept for the life of the process.
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tune;
+
+int ceil_log2(long long v) {
+  int c = 0;
+  while ((1LL << c) < v) ++c;
+  return c;
+}
+
+int size_class(const Sys& s) { return ceil_log2((long long)s.nx * s.ny); }
+
+void order_into(std::vector<Tile>& nat, const Sys& s, int q, int h) {
+  for (const Tile& t : system_tiles(q, s.nx, s.ny, h)) nat.push_back(t);
+}
+
+// Times rounds of (matvec, column update) over the given systems at each candidate height and
+// returns the fastest. The kernels run against scratch inner states (first iteration, alpha = 0):
+// they overwrite the inner work vectors w, p, q and the partial sums, all of which every solve
+// re-initialises before use, and leave x and r as they are; the FP64 solution, the right-hand
+// side and the operator are not touched.
+hipError_t tune_height(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& members, int* best) {
+  const int m = (int)members.size();
+  std::vector<Sys> ts(m);
+  std::vector<char> hin(inner_size(c.fp32) * m, 0);
+  for (int q = 0; q < m; ++q) {
+    if (c.fp32) {
+      Inner<float>* in = reinterpret_cast<Inner<float>*>(hin.data()) + q;
+      in->done = RUN;
+      in->first = 1;
+      in->rho = 1.f;
+    } else {
+      Inner<double>* in = reinterpret_cast<Inner<double>*>(hin.data()) + q;
+      in->done = RUN;
+      in->first = 1;
+      in->rho = 1.0;
+    }
+  }
+  Sys* d_ts = nullptr;
+  void* d_in = nullptr;
+  Tile* d_t = nullptr;
+  hipError_t e = hipMalloc(&d_ts, sizeof(Sys) * m);
+  if (e == hipSuccess) e = hipMalloc(&d_in, hin.size());
+  if (e == hipSuccess) e = hipMemcpy(d_in, hin.data(), hin.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    for (int q = 0; q < m; ++q) {
+      ts[q] = dev[members[q]];
+      ts[q].inner = static_cast<char*>(d_in) + inner_size(c.fp32) * q;
+    }
+    e = hipMemcpy(d_ts, ts.data(), sizeof(Sys) * m, hipMemcpyHostToDevice);
+  }
+  float best_ms = -1.f;
+  *best = 12;
+  for (int hc : kTileHeights) {
+    if (e != hipSuccess) break;
+    std::vector<Tile> nat;
+    for (int q = 0; q < m; ++q) order_into(nat, ts[q], q, hc);
+    std::vector<Tile> ord = xcd_order(nat);
+    const int nt = (int)ord.size();
+    if (d_t) hipFree(d_t);
+    d_t = nullptr;
+    e = hipMalloc(&d_t, sizeof(Tile) * std::max(nt, 1));
+    if (e == hipSuccess && nt) e = hipMemcpy(d_t, ord.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice);
+    // one warm round, then 3 timed rounds
+    float ms = 0.f;
+    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
+      if (rep == 1) e = hipEventRecord(c.ev0, c.stream);
+      if (e == hipSuccess) e = launch_bwd(c.stream, c.fp32, hc, d_ts, d_t, nt, 0);
+      if (e == hipSuccess) e = launch_fwd(c.stream, c.fp32, false, hc, d_ts, d_t, nt);
+    }
+    if (e == hipSuccess) e = hipEventRecord(c.ev1, c.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c.ev0, c.ev1);
+    if (e == hipSuccess && (best_ms < 0.f || ms < best_ms)) {
+      best_ms = ms;
+      *best = hc;
+    }
+  }
+  if (d_t) hipFree(d_t);
+  if (d_ts) hipFree(d_ts);
+  if (d_in) hipFree(d_in);
+  return note_error(e, "tile-height tuning", __LINE__);
+}
+
+// Chooses each system's tile height (forced[q] if nonzero, else tuned per size class) and builds
+// the iteration tile groups, one per height in use, each in XCD order.
+hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& forced) {
+  c.heights.assign(c.nsys, 12);
+  std::map<int, std::vector<int>> classes;
+  for (int q = 0; q < c.nsys; ++q) {
+    if (forced[q] > 0)
+      c.heights[q] = forced[q];
+    else
+      classes[size_class(dev[q])].push_back(q);
+  }
+  for (auto& kv : classes) {
+    const auto key = std::make_tuple(c.device, c.fp32 ? 1 : 0, kv.first, ceil_log2((long long)kv.second.size()));
+    int h = 0;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(key);
+      if (it != g_tune.end()) h = it->second;
+    }
+    if (!h) {
+      HIPCHK(tune_height(c, dev, kv.second, &h));
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = h;
+    }
+    for (int q : kv.second) c.heights[q] = h;
+  }
+  for (int hc : kTileHeights) {
+    std::vector<Tile> nat;
+    for (int q = 0; q < c.nsys; ++q)
+      if (c.heights[q] == hc) order_into(nat, dev[q], q, hc);
+    if (nat.empty()) continue;
+    std::vector<Tile> ord = xcd_order(nat);
+    Group g;
+    g.h = hc;
+    g.ntiles = (int)ord.size();
+    HIPCHK(hipMalloc(&g.d_tiles, sizeof(Tile) * g.ntiles));
+    c.groups.push_back(g);
+    HIPCHK(hipMemcpy(g.d_tiles, ord.data(), sizeof(Tile) * g.ntiles, hipMemcpyHostToDevice));
+  }
+  return hipSuccess;
+}
+
 // Builds a context over the given systems (host Sys copies; outer/inner pointers are replaced by
-// the context's own contiguous arrays).
-hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<std::vector<Tile>>& tiles, bool fp32,
+// the context's own contiguous arrays). forced[q]: tile height of system q, 0 = tuned.
+hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<int>& forced, bool fp32,
                      int check_every, int max_outer, int device, bool prof) {
   c.nsys = (int)systems.size();
   c.prof = prof;
@@ -240,17 +401,12 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   c.device = device;
   for (const Sys& s : systems) c.maxit = std::max(c.maxit, s.maxit);
   std::vector<Tile> nat;
-  for (int q = 0; q < c.nsys; ++q) {
-    for (Tile t : tiles[q]) {
-      t.sys = q;
-      nat.push_back(t);
-    }
-  }
+  for (int q = 0; q < c.nsys; ++q) order_into(nat, systems[q], q, TY);
   std::vector<Tile> ord = xcd_order(nat);
   c.ntiles = (int)ord.size();
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   if (c.prof) {
-    c.kev.assign(3 * (size_t)c.check_every, nullptr);
+    c.kev.assign(4 * (size_t)c.check_every, nullptr);
     for (hipEvent_t& e : c.kev) HIPCHK(hipEventCreate(&e));
   }
   HIPCHK(hipEventCreate(&c.ev0));
@@ -270,19 +426,18 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   HIPCHK(hipMemcpy(c.d_sys, dev.data(), sizeof(Sys) * c.nsys, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&c.d_tiles, sizeof(Tile) * std::max(c.ntiles, 1)));
   if (c.ntiles) HIPCHK(hipMemcpy(c.d_tiles, ord.data(), sizeof(Tile) * c.ntiles, hipMemcpyHostToDevice));
-  return hipSuccess;
+  // the memsets above ran on the null stream; the tuning launches go to the context's stream
+  HIPCHK(hipDeviceSynchronize());
+  return ctx_heights(c, dev, forced);
 }
 
 // check_every iterations (matvec + column update) captured once, replayed per chunk. With kernel
-// timing on, the same launches are issued directly with HIP events around each kernel instead.
+// timing on, the same launches are issued directly with HIP events around each phase's kernels.
 hipError_t ctx_chunk(Ctx& c) {
   if (c.prof) {
     for (int it = 0; it < c.check_every; ++it) {
-      HIPCHK(hipEventRecord(c.kev[3 * it], c.stream));
-      HIPCHK(launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
-      HIPCHK(hipEventRecord(c.kev[3 * it + 1], c.stream));
-      HIPCHK(launch_update(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
-      HIPCHK(hipEventRecord(c.kev[3 * it + 2], c.stream));
+      HIPCHK(iter_bwd(c, 0, c.kev[4 * it], c.kev[4 * it + 1]));
+      HIPCHK(iter_fwd(c, false, c.kev[4 * it + 2], c.kev[4 * it + 3]));
     }
     return hipSuccess;
   }
@@ -291,8 +446,8 @@ hipError_t ctx_chunk(Ctx& c) {
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int it = 0; it < c.check_every && e == hipSuccess; ++it) {
-      e = launch_spmv(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys);
-      if (e == hipSuccess) e = launch_update(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys);
+      e = iter_bwd(c, 0);
+      if (e == hipSuccess) e = iter_fwd(c, false);
     }
     hipError_t e2 = hipStreamEndCapture(c.stream, &g);
     if (e != hipSuccess) {
@@ -318,7 +473,7 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
     bool any = false;
     for (int q = 0; q < c.nsys; ++q) any |= (c.h_outer[q].odone == RUN);
     if (!any) break;
-    HIPCHK(launch_init(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
+    HIPCHK(iter_fwd(c, true));
     std::fill(c.iter_seen.begin(), c.iter_seen.end(), 0);
     // every inner solve ends within maxit iterations (device-side NOTCONV); the host bound below
     // only guarantees termination of this loop whatever the device flags say
@@ -338,8 +493,8 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
       if (c.prof) {
         for (int it = 0; it < std::min(worked, c.check_every); ++it) {
           float a = 0.f, b = 0.f;
-          HIPCHK(hipEventElapsedTime(&a, c.kev[3 * it], c.kev[3 * it + 1]));
-          HIPCHK(hipEventElapsedTime(&b, c.kev[3 * it + 1], c.kev[3 * it + 2]));
+          HIPCHK(hipEventElapsedTime(&a, c.kev[4 * it], c.kev[4 * it + 1]));
+          HIPCHK(hipEventElapsedTime(&b, c.kev[4 * it + 2], c.kev[4 * it + 3]));
           c.mv_ms += a;
           c.up_ms += b;
           c.mv_n += 1;
@@ -368,10 +523,11 @@ int outer_status(const Outer& o) {
   }
 }
 
-void fill_stats(const Outer& o, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
+void fill_stats(const Outer& o, int tile_rows, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
   if (!st) return;
   memset(st, 0, sizeof(*st));
   st->status = outer_status(o);
+  st->tile_rows = tile_rows;
   st->outer_iters = o.outer;
   st->inner_iters = o.inner_total;
   st->rel_res = o.rel;
@@ -407,6 +563,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   s.tiles_x = (h->nx + TX - 1) / TX;
   s.tiles_y = (h->ny + TY - 1) / TY;
   s.ntiles = s.tiles_x * s.tiles_y;
+  s.nslots = s.tiles_x * ((h->ny + RG - 1) / RG);
   void* p;
   hipError_t e;
 #define ALLOC(field, bytes, type)              \
@@ -415,12 +572,22 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
     if (e != hipSuccess) return map_err(e);    \
     s.field = reinterpret_cast<type>(p);       \
   } while (0)
-  ALLOC(A64, 7 * NP * sizeof(double), const double*);
-  ALLOC(S, NP * sizeof(double), const double*);
-  ALLOC(C, 7 * NP * es, const void*);
+  const size_t NP1 = (size_t)h->plane * (h->nz + 1);
+  ALLOC(zw, NP1 * sizeof(double), const double*);
+  ALLOC(zp, NP * sizeof(double), const double*);
+  ALLOC(flg, NP * sizeof(uint16_t), const uint16_t*);
+  ALLOC(col, 7 * (size_t)h->plane * sizeof(double), const double*);
+  if (h->fp32) {
+    ALLOC(zw_in, NP1 * sizeof(float), const void*);
+    ALLOC(zp_in, NP * sizeof(float), const void*);
+    ALLOC(col_in, 7 * (size_t)h->plane * sizeof(float), const void*);
+  } else {
+    s.zw_in = s.zw;
+    s.zp_in = s.zp;
+    s.col_in = s.col;
+  }
   ALLOC(invD, NP * es, const void*);
   ALLOC(L, NP * es, const void*);
-  ALLOC(wet, NP, const uint8_t*);
   ALLOC(smaxp, sizeof(unsigned long long), const double*);
 #undef ALLOC
   h->d_smax = reinterpret_cast<unsigned long long*>(const_cast<double*>(s.smaxp));
@@ -440,7 +607,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   Tile* d_tiles1 = nullptr;
   int* d_bad = nullptr;
   int bad = 0, rc = ORCTM_NH_OK;
-  std::vector<Tile> tl = system_tiles(0, h->nx, h->ny);
+  std::vector<Tile> tl = system_tiles(0, h->nx, h->ny, TY);
   e = hipMalloc(&d_dlxp, n2 * 8);
   if (e == hipSuccess) e = hipMalloc(&d_dlyp, n2 * 8);
   if (e == hipSuccess) e = hipMalloc(&d_dlxu, nu * 8);
@@ -499,8 +666,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   hipFree(d_tiles1);
   hipFree(d_bad);
   if (rc != ORCTM_NH_OK) return rc;
-  h->tiles = tl;
-  e = ctx_build(h->ctx, {s}, {tl}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
+  e = ctx_build(h->ctx, {s}, {h->opts.tile_rows}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
                 h->opts.profile != 0);
   return map_err(e);
 }
@@ -567,6 +733,7 @@ const char* orctm_nh_status_string(int s) {
     case ORCTM_NH_ERR_NOMEM: return "out of device memory";
     case ORCTM_NH_ERR_HIP: return "HIP runtime error";
     case ORCTM_NH_ERR_NODEVICE: return "no usable gfx950 device";
+    case ORCTM_NH_PENDING: return "pending";
     default: return "unknown status";
   }
 }
@@ -584,6 +751,7 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   if (o.max_outer <= 0) o.max_outer = 40;
   if (o.check_every <= 0) o.check_every = 16;
   if (!(o.rtol > 0)) o.rtol = 1e-10;
+  if (o.tile_rows != 0 && o.tile_rows != 4 && o.tile_rows != 8 && o.tile_rows != 12) return ORCTM_NH_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device || o.device < 0) return ORCTM_NH_ERR_NODEVICE;
   if (hipSetDevice(o.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
@@ -662,7 +830,7 @@ int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
   float ms = 0.f;
   hipError_t e = ctx_solve(h->ctx, &ms);
   if (e != hipSuccess) return map_err(e);
-  fill_stats(h->ctx.h_outer[0], st, ms, now_ms() - t0);
+  fill_stats(h->ctx.h_outer[0], h->ctx.heights[0], st, ms, now_ms() - t0);
   return outer_status(h->ctx.h_outer[0]);
 }
 
@@ -722,11 +890,15 @@ int orctm_nh_get_operator(orctm_nh_t* h, double* coef7) {
   std::lock_guard<std::mutex> lk(h->mu);
   hipSetDevice(h->opts.device);
   const size_t NP = (size_t)h->plane * h->nz, N = (size_t)h->nx * h->ny * h->nz;
-  for (int q = 0; q < 7; ++q) {
-    hipError_t e = to_host_vec(h, coef7 + q * N, h->sys.A64 + q * NP, sizeof(double));
-    if (e != hipSuccess) return map_err(e);
-  }
-  return ORCTM_NH_OK;
+  double* dA = nullptr;
+  hipError_t e = hipMalloc(&dA, 7 * NP * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(dA, 0, 7 * NP * sizeof(double));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = launch_materialize(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dA);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  for (int q = 0; q < 7 && e == hipSuccess; ++q) e = to_host_vec(h, coef7 + q * N, dA + q * NP, sizeof(double));
+  hipFree(dA);
+  return map_err(e);
 }
 
 int orctm_nh_get_dims(orctm_nh_t* h, int32_t* nx, int32_t* ny, int32_t* nz) {
@@ -768,7 +940,7 @@ int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
       e = hipMemcpy(c.d_inner, &in, sizeof(in), hipMemcpyHostToDevice);
     }
   }
-  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1, 1);
+  if (e == hipSuccess) e = iter_bwd(c, 1);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   if (e == hipSuccess) e = to_host_vec(h, y, h->sys.q, es);
   return map_err(e);
@@ -786,8 +958,8 @@ int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
   o.odone = RUN;
   hipError_t e = hipMemcpy(c.d_outer, &o, sizeof(o), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = to_device_vec(h, h->sys.r, r, es);
-  if (e == hipSuccess) e = launch_init(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1);
-  if (e == hipSuccess) e = launch_spmv(c.stream, h->fp32, c.d_sys, c.d_tiles, c.ntiles, 1, 0);
+  if (e == hipSuccess) e = iter_fwd(c, true);
+  if (e == hipSuccess) e = iter_bwd(c, 0);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   if (e == hipSuccess) e = to_host_vec(h, z, h->sys.pb, es);
   if (e == hipSuccess && rho) {
@@ -802,6 +974,12 @@ int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
     }
   }
   return map_err(e);
+}
+
+int orctm_nh_state_bytes(orctm_nh_t* h, int64_t* bytes) {
+  if (!h || !bytes) return ORCTM_NH_ERR_INVALID;
+  *bytes = (int64_t)h->bytes;
+  return ORCTM_NH_OK;
 }
 
 int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
@@ -822,7 +1000,6 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
   h->pitch = src->pitch;
   h->plane = src->plane;
   h->fp32 = src->fp32;
-  h->tiles = src->tiles;
   h->sys = src->sys;  // operator pointers shared; state pointers replaced below
   h->d_smax = src->d_smax;
   int rc = alloc_state(h);
@@ -832,8 +1009,8 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
     e = hipMemcpy(h->sys.b, src->sys.b, NP * sizeof(double), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->sys.nbp, src->sys.nbp, sizeof(double), hipMemcpyDeviceToDevice);
     if (e == hipSuccess)
-      e = ctx_build(h->ctx, {h->sys}, {h->tiles}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
-                    h->opts.profile != 0);
+      e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, h->fp32, h->opts.check_every, h->opts.max_outer,
+                    h->opts.device, h->opts.profile != 0);
     rc = map_err(e);
   }
   if (rc != ORCTM_NH_OK) {
@@ -891,14 +1068,14 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
   int max_outer = 0, check = 16, dev = 0;
   bool prof = false;
   std::vector<Sys> ss;
-  std::vector<std::vector<Tile>> tl;
+  std::vector<int> forced;
   for (int q = 0; q < n; ++q) {
     orctm_nh_t* h = systems[q];
     if (!h || h->fp32 != fp32 || h->opts.device != systems[0]->opts.device) return ORCTM_NH_ERR_INVALID;
     for (int r = 0; r < q; ++r)
       if (systems[r] == h) return ORCTM_NH_ERR_INVALID;
     ss.push_back(h->sys);
-    tl.push_back(h->tiles);
+    forced.push_back(h->opts.tile_rows);
     max_outer = std::max(max_outer, h->opts.max_outer);
     check = systems[0]->opts.check_every;
     dev = h->opts.device;
@@ -908,7 +1085,7 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
   if (!b) return ORCTM_NH_ERR_NOMEM;
   b->members.assign(systems, systems + n);
   hipSetDevice(dev);
-  hipError_t e = ctx_build(b->ctx, ss, tl, fp32, check, max_outer, dev, prof);
+  hipError_t e = ctx_build(b->ctx, ss, forced, fp32, check, max_outer, dev, prof);
   if (e != hipSuccess) {
     ctx_destroy(b->ctx);
     delete b;
@@ -937,7 +1114,7 @@ int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
   for (int q = 0; q < b->ctx.nsys; ++q) {
     const int s = outer_status(b->ctx.h_outer[q]);
     if (rc == ORCTM_NH_OK && s != ORCTM_NH_OK) rc = s;
-    if (st) fill_stats(b->ctx.h_outer[q], st + q, ms, tot);
+    if (st) fill_stats(b->ctx.h_outer[q], b->ctx.heights[q], st + q, ms, tot);
   }
   return rc;
 }

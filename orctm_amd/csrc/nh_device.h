@@ -2,9 +2,10 @@
 //
 // Layout in HBM (one system): the interior nx x ny x nz cells of the reference DMDA
 // (mo_petsc.f90:48-50,75-95) stored i-fastest with a row pitch padded to a multiple of 32
-// elements: cell (i,j,k) at k*plane + j*pitch + i, plane = pitch*ny. Coefficients are 7 such
-// planes-of-volumes (centre, W, E, N, S, U, D), each nz*plane long. Padding cells are never read
-// as data: every neighbour access goes through the wrap/validity map of the domain.
+// elements: cell (i,j,k) at k*plane + j*pitch + i, plane = pitch*ny. The operator is kept in a
+// compact form (ZW, ZP per cell, 12 flag bits per cell, 7 metric products per column) from which
+// every row of compute_matrix is rebuilt on the fly with the assembly's own operations. Padding
+// cells are never read as data: every neighbour access goes through the wrap/validity map.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,9 +17,10 @@ constexpr int TX = 64;           // tile width in i: one wavefront, consecutive 
 #ifndef ORCTM_TILE_Y
 #define ORCTM_TILE_Y 12
 #endif
-constexpr int TY = ORCTM_TILE_Y; // tile height in j: one wavefront per row
-static_assert(TY >= 4 && TY <= 16, "the matvec halo needs waves 0-2 (TY >= 4); blocks hold <= 1024 threads");
-constexpr int NT = TX * TY;      // threads per block
+constexpr int TY = ORCTM_TILE_Y; // tile height in j of the set-up kernels: one wavefront per row
+constexpr int RG = 4;            // rows per reduction group (block partial sums are per 4 rows)
+static_assert(TY % RG == 0 && TY <= 16, "tiles hold whole reduction groups; blocks hold <= 1024 threads");
+constexpr int NT = TX * TY;      // threads per block of the set-up kernels
 constexpr int NCOEF = 7;
 
 enum Done : int { RUN = 0, CONVERGED = 1, NOTCONV = 2, BREAKDOWN = 3, NONFINITE = 4, SKIP = 5 };
@@ -49,13 +51,18 @@ struct Sys {
   int nx, ny, nz, pitch;
   long long plane;
   int cyc_x, cyc_y;
-  int tiles_x, tiles_y, ntiles, pad0;
-  const double* A64;    // reference operator, 7 planes
-  const double* S;      // symmetrising weight DLXP*DLYP/ZP
-  const void* C;        // inner operator S*(-A) in T, 7 planes
+  int tiles_x, tiles_y, ntiles;  // set-up tiling (64 x TY)
+  int nslots;                    // partial-sum slots: tiles_x * ceil(ny / 4), one per 64 x 4 group
+  // operator in compact form (rows are rebuilt on the fly, see row_coef in nh_kernels.hip)
+  const double* zw;     // ZW, nz+1 planes (mo_petsc.f90:557-583)
+  const double* zp;     // ZP, nz planes (mo_petsc.f90:585-590)
+  const uint16_t* flg;  // per-cell flag bits: wet, couplings present, X_C/Y_C/Z_C variants
+  const double* col;    // 7 per-column planes: XUw*XP, XUe*XP, YVn*YP, YVs*YP, X_C, Y_C, DLXP*DLYP
+  const void* zw_in;    // the same in the inner precision (FP32 copies for MIXED; aliases for FP64)
+  const void* zp_in;
+  const void* col_in;
   const void* invD;     // line preconditioner: 1/D_k
   const void* L;        // line preconditioner: l_k
-  const uint8_t* wet;   // WETO as 0/1
   double* b;            // right-hand side DIVG*WETO
   double* p64;          // solution (FP64)
   void* x;              // inner solution
@@ -64,7 +71,7 @@ struct Sys {
   void* pa;             // search direction, buffer A
   void* pb;             // search direction, buffer B
   void* q;              // C p
-  double* partials;     // 2*ntiles slots of block partial sums (summed by the reduction kernel)
+  double* partials;     // 2*nslots partial sums (summed by the reduction kernel in slot order)
   Outer* outer;
   void* inner;          // Inner<T>*
   double* nbp;          // ||b||_2 of the resident right-hand side (written by the rhs kernel)
@@ -76,7 +83,7 @@ struct Sys {
 struct Tile {
   int sys;    // index into the Sys array of the launch
   int bx, by; // tile coordinates within the system
-  int local;  // tile index within its system (partials slot)
+  int local;  // tile index within its system
 };
 
 }  // namespace onh

@@ -14,7 +14,8 @@ struct FortranIn {
   const double *dlxp, *dlyp, *dlxu, *dlyv, *ddpo, *weto, *dzw;
 };
 
-// Assembly: A64, S, C (inner precision), wet; flags invalid input in *bad; max S in *smax_bits.
+// Assembly into the compact form (ZW, ZP, flags, column products and their inner-precision copies);
+// flags invalid input in *bad; max S in *smax_bits.
 hipError_t launch_assemble(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles,
                            const FortranIn& f, int* d_bad, unsigned long long* d_smax_bits);
 hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
@@ -31,12 +32,18 @@ hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, i
 hipError_t launch_begin(hipStream_t st, bool fp32, const Sys* d_sys, int nsys);
 // Each phase launch below is followed by its one-block-per-system reduction (nsys blocks).
 hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
-hipError_t launch_init(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
+// Inner iteration kernels on tiles of 64 x h columns (h in kTileHeights; d_tiles built for that h).
+// Each launch over one group of tiles; the per-system reduction of the phase follows once all
+// groups of the phase have been launched. init: x = 0 and the first forward elimination.
+constexpr int kTileHeights[3] = {4, 8, 12};
+enum InnerPhase { PH_INIT = 0, PH_BWD = 1, PH_FWD = 2 };
+hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles);
 // raw = 1 (verification only): q = C w without the back substitution.
-hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
-                       int raw = 0);
-hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
+hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw);
+hipError_t launch_inner_reduce(hipStream_t st, bool fp32, InnerPhase ph, const Sys* d_sys, int nsys);
 hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+// The 7 planes of the reference operator rebuilt from the compact form (verification entry point).
+hipError_t launch_materialize(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, double* A);
 // dst <- src, bytes rounded down to 16 (bandwidth probe).
 hipError_t launch_copy4(hipStream_t st, const void* src, void* dst, size_t bytes);
 // y = A64 x on padded interior vectors (verification entry point).

@@ -62,15 +62,36 @@ __device__ __forceinline__ void block_sum2(T& a, T& b) {
   }
 }
 
-// Publishes this block's (a, b) (valid in thread 0) into its system's partials slot. The system
-// totals are formed by k_reduce, launched right after, in tile order: no atomics, no fences (a
-// per-block device-scope fence costs about a third of the bandwidth of these streaming kernels).
-template <typename T>
-__device__ __forceinline__ void publish2(const Sys& s, int local, T a, T b) {
-  if (threadIdx.x == 0) {
-    gptr<T> part = G<T>(s.partials);
-    part[2 * local] = a;
-    part[2 * local + 1] = b;
+// Publishes the per-row sums of (a, b) of a block whose wavefront w (w < H) holds row j0 + w of its
+// tile, as totals over fixed groups of RG = 4 consecutive rows: group g = j / 4 of x-tile bx goes to
+// slot g * tiles_x + bx of the system's partials, as ((r0 + r1) + (r2 + r3)) of its row sums. The
+// system totals, formed by k_reduce over the slots in a fixed order, are therefore the same bits for
+// every tile height H (any multiple of 4), so the tile shape can be tuned without changing results.
+// No atomics and no fences (a per-block device-scope fence costs about a third of the bandwidth of
+// these streaming kernels). Wavefronts w >= H (halo waves) contribute nothing.
+template <typename T, int H, int NW>
+__device__ __forceinline__ void publish_rows(const Sys& s, int bx, int by, T a, T b) {
+  static_assert(H % RG == 0 && H <= NW, "row groups must not straddle tiles");
+  __shared__ T sh[2 * H];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0 && w < H) {
+    sh[w] = a;
+    sh[H + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < H / RG) {
+    const int g = threadIdx.x, grow = by * (H / RG) + g;
+    if (grow * RG < s.ny) {
+      const T sa = (sh[4 * g] + sh[4 * g + 1]) + (sh[4 * g + 2] + sh[4 * g + 3]);
+      const T sb = (sh[H + 4 * g] + sh[H + 4 * g + 1]) + (sh[H + 4 * g + 2] + sh[H + 4 * g + 3]);
+      gptr<T> part = G<T>(s.partials);
+      const size_t slot = (size_t)grow * s.tiles_x + bx;
+      part[2 * slot] = sa;
+      part[2 * slot + 1] = sb;
+    }
   }
 }
 
@@ -116,6 +137,28 @@ __device__ __forceinline__ double dddwo(const FortranIn& f, int I, int J, int K)
   return (dm + d0) / 2.0;
 }
 
+// One row of compute_matrix rebuilt from the compact form, in T arithmetic (double: the operations
+// of the assembly, bit for bit; float: the inner operator). c7: centre, W, E, N, S, U, D.
+// flags: bit 0 wet; bits 1-6 coupling present W E N S U D; bits 7-8 X_C variant (full, XUe*XP,
+// XUw*XP, 0); bits 9-10 Y_C variant (full, YVs*YP, YVn*YP, 0); bit 11 Z_C variant (full, ZW*ZP).
+template <typename T>
+__device__ __forceinline__ void row_coef(const T* cc, T zwk, T zwk1, T zpk, unsigned f, T* c7) {
+  const T xw = cc[0], xe = cc[1], yn = cc[2], ys = cc[3];
+  const unsigned xc = (f >> 7) & 3u, yc = (f >> 9) & 3u;
+  const T X_C = xc == 0 ? cc[4] : xc == 1 ? xe : xc == 2 ? xw : T(0);
+  const T Y_C = yc == 0 ? cc[5] : yc == 1 ? ys : yc == 2 ? yn : T(0);
+  const T Z_C = ((f >> 11) & 1u) ? zwk * zpk : (zwk + zwk1) * zpk;
+  c7[0] = -(X_C + Y_C + Z_C);
+  c7[1] = (f & (1u << 1)) ? xw : T(0);
+  c7[2] = (f & (1u << 2)) ? xe : T(0);
+  c7[3] = (f & (1u << 3)) ? yn : T(0);
+  c7[4] = (f & (1u << 4)) ? ys : T(0);
+  c7[5] = (f & (1u << 5)) ? zwk * zpk : T(0);
+  c7[6] = (f & (1u << 6)) ? zwk1 * zpk : T(0);
+}
+
+// Assembly into the compact form. ZW/ZP follow mo_petsc.f90:557-590, the per-column products
+// mo_petsc.f90:593-599,610-615, the flag bits the branches of mo_petsc.f90:623-765.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_assemble(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                  FortranIn f, int* bad, unsigned long long* smax_bits) {
@@ -127,90 +170,77 @@ __global__ __launch_bounds__(NT) void k_assemble(const Sys* __restrict__ systems
   int nbad = 0;
   if (active) {
     const int I = i + 2, J = j + 2, IE1 = f.IE - 1, JE1 = f.JE - 1, KE = f.KE;
-    const size_t NP = (size_t)s.plane * s.nz;
-    double* A = const_cast<double*>(s.A64);
-    double* S = const_cast<double*>(s.S);
-    T* C = reinterpret_cast<T*>(const_cast<void*>(s.C));
-    uint8_t* wet = const_cast<uint8_t*>(s.wet);
+    const size_t P = s.plane, col = (size_t)j * s.pitch + i;
+    gptr<double> zw = G<double>(s.zw);
+    gptr<double> zp = G<double>(s.zp);
+    gptr<uint16_t> flg = G<uint16_t>(s.flg);
+    gptr<double> cp = G<double>(s.col);
+    gptr<T> zwi = G<T>(s.zw_in);
+    gptr<T> zpi = G<T>(s.zp_in);
+    gptr<T> cpi = G<T>(s.col_in);
+    const bool copy_in = (const void*)s.zw_in != (const void*)s.zw;
     const double dlxuw = F2U(f.dlxu, I - 1, J), dlxue = F2U(f.dlxu, I, J);
     const double dlyvn = F2V(f.dlyv, I, J - 1), dlyvs = F2V(f.dlyv, I, J);
     const double dlxp = F2P(f.dlxp, I, J), dlyp = F2P(f.dlyp, I, J);
     if (!(dlxuw > 0) || !(dlxue > 0) || !(dlyvn > 0) || !(dlyvs > 0) || !(dlxp > 0) || !(dlyp > 0)) nbad = 1;
     const double xuw = 1.0 / dlxuw, xue = 1.0 / dlxue, xp = 1.0 / dlxp;
     const double yvn = 1.0 / dlyvn, yvs = 1.0 / dlyvs, yp = 1.0 / dlyp;
+    const double cc[7] = {xuw * xp, xue * xp, yvn * yp, yvs * yp, (xuw + xue) * xp, (yvn + yvs) * yp, dlxp * dlyp};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      cp[q * P + col] = cc[q];
+      if (copy_in) cpi[q * P + col] = (T)cc[q];
+    }
     double zwK = 1.0 / dddwo(f, I, J, 1);
+    zw[col] = zwK;
+    if (copy_in) zwi[col] = (T)zwK;
     for (int K = 1; K <= KE; ++K) {
+      const size_t idx = (size_t)(K - 1) * P + col;
       const double zwK1 = 1.0 / dddwo(f, I, J, K + 1);
-      const double zp = 1.0 / ((1.0 / zwK + 1.0 / zwK1) / 2.0);
-      if (!isfinite(zwK) || !isfinite(zwK1) || !isfinite(zp) || !(zp > 0)) nbad = 1;
-      double X_W = xuw * xp, X_C = (xuw + xue) * xp, X_E = xue * xp;
-      double Y_N = yvn * yp, Y_C = (yvn + yvs) * yp, Y_S = yvs * yp;
-      double Z_U = zwK * zp, Z_C = (zwK + zwK1) * zp, Z_D = zwK1 * zp;
-      double c[NCOEF] = {0, 0, 0, 0, 0, 0, 0};
+      const double zpK = 1.0 / ((1.0 / zwK + 1.0 / zwK1) / 2.0);
+      if (!isfinite(zwK) || !isfinite(zwK1) || !isfinite(zpK) || !(zpK > 0)) nbad = 1;
+      zw[idx + P] = zwK1;
+      zp[idx] = zpK;
+      if (copy_in) {
+        zwi[idx + P] = (T)zwK1;
+        zpi[idx] = (T)zpK;
+      }
       const double w0 = F3(f.weto, I, J, K);
       if (w0 != 0.0 && w0 != 1.0) nbad = 1;
-      if (w0 == 0.0) {
-        c[0] = -(X_C + Y_C + Z_C);  // land row, mo_petsc.f90:623-626
-      } else {
+      unsigned fl = 0;
+      if (w0 != 0.0) {
         const double ww = F3(f.weto, I - 1, J, K), we = F3(f.weto, I + 1, J, K);
         const double wn = F3(f.weto, I, J - 1, K), ws = F3(f.weto, I, J + 1, K);
-        if (ww == 0.0 && we == 1.0) {  // mo_petsc.f90:630-640
-          X_W = 0.0;
-          X_C = xue * xp;
-        } else if (ww == 1.0 && we == 0.0) {
-          X_E = 0.0;
-          X_C = xuw * xp;
-        } else if (ww == 0.0 && we == 0.0) {
-          X_W = 0.0;
-          X_E = 0.0;
-          X_C = 0.0;
-        }
-        if (wn == 0.0 && ws == 1.0) {  // mo_petsc.f90:643-653
-          Y_N = 0.0;
-          Y_C = yvs * yp;
-        } else if (wn == 1.0 && ws == 0.0) {
-          Y_S = 0.0;
-          Y_C = yvn * yp;
-        } else if (wn == 0.0 && ws == 0.0) {
-          Y_N = 0.0;
-          Y_S = 0.0;
-          Y_C = 0.0;
-        }
-        if (K == 1) {  // mo_petsc.f90:657-665
-          Z_U = 0.0;
-        } else if (K == KE) {
-          Z_D = 0.0;
-          Z_C = zwK * zp;
-        } else if (F3(f.weto, I, J, K + 1) == 0.0) {
-          Z_D = 0.0;
-          Z_C = zwK * zp;
-        }
-        if (!f.cyc_x) {  // mo_petsc.f90:669-678
-          if (f.open_w && I == 2) X_W = 0.0;
-          if (f.open_e && I == IE1) X_E = 0.0;
-        }
-        if (!f.cyc_y) {  // mo_petsc.f90:680-689
-          if (f.open_n && J == 2) Y_N = 0.0;
-          if (f.open_s && J == JE1) Y_S = 0.0;
-        }
-        c[0] = -(X_C + Y_C + Z_C);
-        if (ww == 1.0 && (f.cyc_x || I != 2)) c[1] = X_W;  // mo_petsc.f90:692-764
-        if (we == 1.0 && (f.cyc_x || I != IE1)) c[2] = X_E;
-        if (wn == 1.0 && (f.cyc_y || J != 2)) c[3] = Y_N;
-        if (ws == 1.0 && (f.cyc_y || J != JE1)) c[4] = Y_S;
-        if (K != 1) c[5] = Z_U;
-        if (K != KE && F3(f.weto, I, J, K + 1) == 1.0) c[6] = Z_D;
+        fl |= 1u;
+        if (ww == 1.0 && (f.cyc_x || I != 2)) fl |= 1u << 1;  // couplings, mo_petsc.f90:692-764
+        if (we == 1.0 && (f.cyc_x || I != IE1)) fl |= 1u << 2;
+        if (wn == 1.0 && (f.cyc_y || J != 2)) fl |= 1u << 3;
+        if (ws == 1.0 && (f.cyc_y || J != JE1)) fl |= 1u << 4;
+        if (K != 1) fl |= 1u << 5;
+        if (K != KE && F3(f.weto, I, J, K + 1) == 1.0) fl |= 1u << 6;
+        unsigned xc = 0, yc = 0, zc = 0;  // Neumann at land, mo_petsc.f90:630-653
+        if (ww == 0.0 && we == 1.0)
+          xc = 1;
+        else if (ww == 1.0 && we == 0.0)
+          xc = 2;
+        else if (ww == 0.0 && we == 0.0)
+          xc = 3;
+        if (wn == 0.0 && ws == 1.0)
+          yc = 1;
+        else if (wn == 1.0 && ws == 0.0)
+          yc = 2;
+        else if (wn == 0.0 && ws == 0.0)
+          yc = 3;
+        // surface / bottom, mo_petsc.f90:657-665 (K == 1 is tested first)
+        if (K == 1)
+          zc = 0;
+        else if (K == KE || F3(f.weto, I, J, K + 1) == 0.0)
+          zc = 1;
+        fl |= xc << 7 | yc << 9 | zc << 11;
       }
-      const size_t idx = (size_t)(K - 1) * s.plane + (size_t)j * s.pitch + i;
-      const double Sv = (dlxp * dlyp) / zp;
-#pragma unroll
-      for (int q = 0; q < NCOEF; ++q) {
-        A[q * NP + idx] = c[q];
-        C[q * NP + idx] = (T)(Sv * (-c[q]));
-      }
-      S[idx] = Sv;
+      flg[idx] = (uint16_t)fl;
+      const double Sv = cc[6] / zpK;
       smax = fmax(smax, Sv);
-      wet[idx] = (w0 == 1.0) ? 1 : 0;
       zwK = zwK1;
     }
   }
@@ -239,31 +269,48 @@ __global__ __launch_bounds__(NT) void k_assemble(const Sys* __restrict__ systems
   }
 }
 
-// Line preconditioner factorisation (per column), restated by orc_line_factor*.
+// Inner operator row C = S * (-A), S = DLXP*DLYP/ZP, in T arithmetic from the T copies.
+template <typename T>
+__device__ __forceinline__ void inner_row(const T* cc, T zwk, T zwk1, T zpk, unsigned f, T* c7) {
+  row_coef<T>(cc, zwk, zwk1, zpk, f, c7);
+  const T S = cc[6] / zpk;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) c7[q] = S * (-c7[q]);
+}
+
+// Line preconditioner factorisation (per column), restated by orc_line_factor*: the tridiagonal of
+// the inner operator (centre, and the up coupling of wet-wet pairs as the symmetric off-diagonal).
 template <typename T>
 __global__ __launch_bounds__(NT) void k_line_factor(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   if (i >= s.nx || j >= s.ny) return;
-  const size_t P = s.plane, NP = P * s.nz, col = (size_t)j * s.pitch + i;
-  const T* Cc = reinterpret_cast<const T*>(s.C);
-  const T* Cu = Cc + 5 * NP;
-  const T* Cd = Cc + 6 * NP;
-  T* invD = reinterpret_cast<T*>(const_cast<void*>(s.invD));
-  T* L = reinterpret_cast<T*>(const_cast<void*>(s.L));
-  T Dprev = T(1);
+  const size_t P = s.plane, col = (size_t)j * s.pitch + i;
+  gptr<const T> zwi = G<const T>(s.zw_in);
+  gptr<const T> zpi = G<const T>(s.zp_in);
+  gptr<const T> cpi = G<const T>(s.col_in);
+  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+  gptr<T> invD = G<T>(s.invD);
+  gptr<T> L = G<T>(s.L);
+  T cc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) cc[q] = cpi[q * P + col];
+  T Dprev = T(1), cd_prev = T(0);
   for (int k = 0; k < s.nz; ++k) {
     const size_t n = col + (size_t)k * P;
-    T d = Cc[n], l = T(0);
+    T c7[7];
+    inner_row<T>(cc, zwi[n], zwi[n + P], zpi[n], flg[n], c7);
+    T d = c7[0], l = T(0);
     if (k > 0) {
-      const T e = (Cd[n - P] != T(0) && Cu[n] != T(0)) ? Cu[n] : T(0);
+      const T e = (cd_prev != T(0) && c7[5] != T(0)) ? c7[5] : T(0);
       l = e / Dprev;
       d = fmad(-l, e, d);
     }
     invD[n] = T(1) / d;
     L[n] = l;
     Dprev = d;
+    cd_prev = c7[6];
   }
 }
 
@@ -280,13 +327,13 @@ __global__ __launch_bounds__(NT) void k_rhs(const Sys* __restrict__ systems, con
     const size_t col = (size_t)j * s.pitch + i;
     for (int k = 0; k < s.nz; ++k) {
       const size_t idx = col + (size_t)k * s.plane;
-      const double v = divg[(size_t)(i + 1) + (size_t)IE * ((size_t)(j + 1) + (size_t)JE * k)] * (double)s.wet[idx];
+      const double wet = (s.flg[idx] & 1u) ? 1.0 : 0.0;
+      const double v = divg[(size_t)(i + 1) + (size_t)IE * ((size_t)(j + 1) + (size_t)JE * k)] * wet;
       s.b[idx] = v;
       acc = fma(v, v, acc);
     }
   }
-  block_sum2(acc, dummy);
-  publish2(s, t.local, acc, dummy);
+  publish_rows<double, TY, TY>(s, t.bx, t.by, acc, dummy);
 }
 
 // b <- b * (1 + amplitude * xi), xi ~ N(0,1) per cell from a counter-based SplitMix64 stream
@@ -320,8 +367,7 @@ __global__ __launch_bounds__(NT) void k_perturb(const Sys* __restrict__ systems,
       acc = fma(v, v, acc);
     }
   }
-  block_sum2(acc, dummy);
-  publish2(s, t.local, acc, dummy);
+  publish_rows<double, TY, TY>(s, t.bx, t.by, acc, dummy);
 }
 
 __global__ __launch_bounds__(NT) void k_guess(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
@@ -338,7 +384,7 @@ __global__ __launch_bounds__(NT) void k_guess(const Sys* __restrict__ systems, c
 // ------------------------------------------------------------------------------------------------
 // FP64 reference matvec, restated by orc_matvec64: centre, W, E, N, S, U, D, absent = +0.
 
-__device__ __forceinline__ double apply_a64(const Sys& s, gptr<const double> x, size_t NP, int i, int j, int k,
+__device__ __forceinline__ double apply_a64(const Sys& s, const double* cc, gptr<const double> x, int i, int j, int k,
                                             size_t idx) {
   const size_t IP = s.pitch, P = s.plane;
   const int nx = s.nx, ny = s.ny;
@@ -355,15 +401,23 @@ __device__ __forceinline__ double apply_a64(const Sys& s, gptr<const double> x, 
   const double vw = x[iw], ve = x[ie], vn = x[in_], vs = x[is], vu = x[iu], vd = x[id];
   const double xw = hw ? vw : 0.0, xe = he ? ve : 0.0, xn = hn ? vn : 0.0, xs = hs ? vs : 0.0;
   const double xu = k > 0 ? vu : 0.0, xd = k < s.nz - 1 ? vd : 0.0;
-  gptr<const double> A = G<const double>(s.A64);
-  double acc = A[idx] * x0;
-  acc = fma(A[NP + idx], xw, acc);
-  acc = fma(A[2 * NP + idx], xe, acc);
-  acc = fma(A[3 * NP + idx], xn, acc);
-  acc = fma(A[4 * NP + idx], xs, acc);
-  acc = fma(A[5 * NP + idx], xu, acc);
-  acc = fma(A[6 * NP + idx], xd, acc);
+  double c7[7];
+  row_coef<double>(cc, G<const double>(s.zw)[idx], G<const double>(s.zw)[idx + P], G<const double>(s.zp)[idx],
+                   G<const uint16_t>(s.flg)[idx], c7);
+  double acc = c7[0] * x0;
+  acc = fma(c7[1], xw, acc);
+  acc = fma(c7[2], xe, acc);
+  acc = fma(c7[3], xn, acc);
+  acc = fma(c7[4], xs, acc);
+  acc = fma(c7[5], xu, acc);
+  acc = fma(c7[6], xd, acc);
   return acc;
+}
+
+__device__ __forceinline__ void load_col(const Sys& s, size_t col, double* cc) {
+  gptr<const double> cp = G<const double>(s.col);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) cc[q] = cp[q * (size_t)s.plane + col];
 }
 
 __global__ __launch_bounds__(NT) void k_apply64(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
@@ -372,10 +426,30 @@ __global__ __launch_bounds__(NT) void k_apply64(const Sys* __restrict__ systems,
   const Sys& s = systems[t.sys];
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   if (i >= s.nx || j >= s.ny) return;
-  const size_t NP = (size_t)s.plane * s.nz, col = (size_t)j * s.pitch + i;
+  const size_t col = (size_t)j * s.pitch + i;
+  double cc[7];
+  load_col(s, col, cc);
   for (int k = 0; k < s.nz; ++k) {
     const size_t idx = col + (size_t)k * s.plane;
-    y[idx] = apply_a64(s, G<const double>(x), NP, i, j, k, idx);
+    y[idx] = apply_a64(s, cc, G<const double>(x), i, j, k, idx);
+  }
+}
+
+// Materialises the 7 planes of the reference operator (verification entry point).
+__global__ __launch_bounds__(NT) void k_materialize(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                    double* __restrict__ A) {
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
+  if (i >= s.nx || j >= s.ny) return;
+  const size_t P = s.plane, NP = P * s.nz, col = (size_t)j * s.pitch + i;
+  double cc[7], c7[7];
+  load_col(s, col, cc);
+  for (int k = 0; k < s.nz; ++k) {
+    const size_t idx = col + (size_t)k * P;
+    row_coef<double>(cc, s.zw[idx], s.zw[idx + P], s.zp[idx], s.flg[idx], c7);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) A[q * NP + idx] = c7[q];
   }
 }
 
@@ -418,8 +492,10 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   double acc = 0.0, dummy = 0.0;
   if (i < s.nx && j < s.ny) {
-    const size_t NP = (size_t)s.plane * s.nz, col = (size_t)j * s.pitch + i;
+    const size_t col = (size_t)j * s.pitch + i;
     gptr<T> rin = G<T>(s.r);
+    double cc[7];
+    load_col(s, col, cc);
 #pragma unroll 2
     for (int k = 0; k < s.nz; ++k) {
       const size_t idx = col + (size_t)k * s.plane;
@@ -428,13 +504,13 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         rin[idx] = T(0);
         continue;
       }
-      const double r = G<const double>(s.b)[idx] - apply_a64(s, G<const double>(s.p64), NP, i, j, k, idx);
+      const double r = G<const double>(s.b)[idx] - apply_a64(s, cc, G<const double>(s.p64), i, j, k, idx);
       acc = fma(r, r, acc);
-      rin[idx] = (T)((G<const double>(s.S)[idx] * (-r)) * gamma);
+      const double S = cc[6] / G<const double>(s.zp)[idx];
+      rin[idx] = (T)((S * (-r)) * gamma);
     }
   }
-  block_sum2(acc, dummy);
-  publish2(s, t.local, acc, dummy);
+  publish_rows<double, TY, TY>(s, t.bx, t.by, acc, dummy);
 }
 
 template <typename T>
@@ -462,14 +538,15 @@ __global__ __launch_bounds__(NT) void k_refine(const Sys* __restrict__ systems, 
 //   k_fwd  (per column, top-down):   x += alpha p; r -= alpha q; forward elimination of the line
 //          preconditioner  y_k = r_k - l_k y_{k-1},  w_k = y_k / D_k;  r.z = sum y_k w_k,  r.r.
 //          Reads x p r q l invD, writes x r w: 9 words per cell.
-//   k_bwd  (64x4 tile, bottom-up):   back substitution z_k = w_k - l_{k+1} z_{k+1} fused with
-//          p = z + beta p_old, q = C p (7-point, horizontal neighbours through LDS) and p.q.
-//          Reads w l p_old + 7 coefficients, writes p q: 12 words per cell.
+//   k_bwd  (64 x H tile, bottom-up): back substitution z_k = w_k - l_{k+1} z_{k+1} fused with
+//          p = z + beta p_old, q = C p (7-point, horizontal neighbours through LDS) and p.q; the
+//          rows of C are rebuilt from ZW, ZP, the flag bits and the column's metric products.
+//          Reads w l p_old ZW ZP (+ 2 B of flags), writes p q: 7 words + 2 B per cell.
 // The halo lanes of k_bwd run the back-substitution recurrence of their halo columns themselves, so
 // z never goes to memory and no grid-wide step separates the two halves of the line solve.
 
-template <typename T, bool INIT>
-__global__ __launch_bounds__(NT) void k_fwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
+template <typename T, bool INIT, int H>
+__global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
@@ -479,7 +556,7 @@ __global__ __launch_bounds__(NT) void k_fwd(const Sys* __restrict__ systems, con
   } else {
     if (in->done != RUN) return;
   }
-  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
+  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * H + (threadIdx.x >> 6);
   const size_t P = s.plane, col = (size_t)j * s.pitch + i;
   const int nz = s.nz;
   gptr<T> x = G<T>(s.x);
@@ -547,20 +624,21 @@ __global__ __launch_bounds__(NT) void k_fwd(const Sys* __restrict__ systems, con
       compute(k0 + U, b1);
     }
   }
-  block_sum2(rho, rr);
-  publish2(s, t.local, rho, rr);
+  publish_rows<T, H, H>(s, t.bx, t.by, rho, rr);
 }
 
-// Threads of the matvec kernel: TY wavefronts own the tile rows (one column each, centre slots);
-// three more wavefronts own the halo: the north row, the south row and the 2*TY west/east cells.
-// Every thread runs the back-substitution recurrence of exactly one column, so no wavefront does
-// both centre and halo work between two barriers.
-constexpr int NW_BWD = TY + 3;
-constexpr int NT_BWD = 64 * NW_BWD;
+// Threads of the matvec kernel (tile of 64 x H columns): H wavefronts own the tile rows (one column
+// each, centre slots); three more wavefronts own the halo: the north row, the south row and the 2*H
+// west/east cells. Every thread runs the back-substitution recurrence of exactly one column, so no
+// wavefront does both centre and halo work between two barriers.
+template <int H>
+constexpr int bwd_threads() {
+  return 64 * (H + 3);
+}
 
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
-template <typename T>
-__global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+template <typename T, int H>
+__global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                 int raw) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
@@ -568,7 +646,7 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
   if (in->done != RUN) return;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int i0 = t.bx * TX, j0 = t.by * TY;
+  const int i0 = t.bx * TX, j0 = t.by * H;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const size_t P = s.plane, IP = s.pitch, NP = P * (size_t)nz;
   const bool first = in->first != 0;
@@ -579,32 +657,34 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
   gptr<const T> pold = G<const T>(pb ? s.pb : s.pa);
   gptr<T> pnew = G<T>(pb ? s.pa : s.pb);
   gptr<T> q = G<T>(s.q);
-  gptr<const T> C = G<const T>(s.C);
+  gptr<const T> zwi = G<const T>(s.zw_in);
+  gptr<const T> zpi = G<const T>(s.zp_in);
+  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
 
-  __shared__ T lds[3][TY + 2][TX + 2];
+  __shared__ T lds[3][H + 2][TX + 2];
 
   // this thread's slot in the LDS plane and the column it follows
   int gi, gj, sx, sy;
   bool slot = true;
-  const bool centre = ty < TY;
+  const bool centre = ty < H;
   if (centre) {
     gi = i0 + tx;
     gj = j0 + ty;
     sx = tx + 1;
     sy = ty + 1;
-  } else if (ty == TY) {
+  } else if (ty == H) {
     gi = i0 + tx;
     gj = j0 - 1;
     sx = tx + 1;
     sy = 0;
-  } else if (ty == TY + 1) {
+  } else if (ty == H + 1) {
     gi = i0 + tx;
-    gj = j0 + TY;
+    gj = j0 + H;
     sx = tx + 1;
-    sy = TY + 1;
+    sy = H + 1;
   } else {
-    slot = tx < 2 * TY;
-    const int e = tx < TY ? 0 : 1, r = tx < TY ? tx : tx - TY;
+    slot = tx < 2 * H;
+    const int e = tx < H ? 0 : 1, r = tx < H ? tx : tx - H;
     gi = e ? i0 + TX : i0 - 1;
     gj = j0 + (slot ? r : 0);
     sx = e ? TX + 1 : 0;
@@ -616,12 +696,22 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
   const size_t off = val ? (size_t)mj * IP + mi : 0;
 
   // Software pipeline over levels (bottom-up) in two register sets used alternately: while level k
-  // is finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its
-  // coefficients), set `nxt` receives w/l/p_old of level k-2 and the coefficients of level k-1.
+  // is finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its row
+  // data), set `nxt` receives w/l/p_old of level k-2 and the row data (ZW, ZP, flags) of level k-1.
+  // The 7 inner coefficients are rebuilt from the row data and this column's metric products.
   struct Stage {
     T w, l, p;
-    T c[NCOEF];
+    T zw, zp;
+    unsigned f;
   };
+  T cc[7];
+  if (centre) {
+    gptr<const T> cpi = G<const T>(s.col_in);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) cc[c] = cpi[c * P + off];
+  }
+  T zw_below = T(0);  // ZW at the bottom face of the current level (top face of the level below)
+  (void)NP;
   T zc = T(0), lc = T(0);  // z and l of this thread's column at the level below
   auto load_a = [&](int m, Stage& st) {
     const size_t om = (size_t)(m > 0 ? m : 0) * P + off;
@@ -632,8 +722,9 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
   auto load_c = [&](int m, Stage& st) {
     if (!centre) return;
     const size_t cm = (size_t)(m > 0 ? m : 0) * P + off;
-#pragma unroll
-    for (int c = 0; c < NCOEF; ++c) st.c[c] = C[c * NP + cm];
+    st.zw = zwi[cm];
+    st.zp = zpi[cm];
+    st.f = flg[cm];
   };
   // back substitution and search-direction update of level m for this thread's column
   auto level = [&](int m, const Stage& st) -> T {
@@ -654,13 +745,16 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
     __syncthreads();
     if (centre) {
       const int b0 = k % 3;
-      T acc = cur.c[0] * p0;
-      acc = fmad(cur.c[1], lds[b0][sy][sx - 1], acc);
-      acc = fmad(cur.c[2], lds[b0][sy][sx + 1], acc);
-      acc = fmad(cur.c[3], lds[b0][sy - 1][sx], acc);
-      acc = fmad(cur.c[4], lds[b0][sy + 1][sx], acc);
-      acc = fmad(cur.c[5], pm1, acc);
-      acc = fmad(cur.c[6], pp1, acc);
+      T c7[7];
+      inner_row<T>(cc, cur.zw, zw_below, cur.zp, cur.f, c7);
+      zw_below = cur.zw;
+      T acc = c7[0] * p0;
+      acc = fmad(c7[1], lds[b0][sy][sx - 1], acc);
+      acc = fmad(c7[2], lds[b0][sy][sx + 1], acc);
+      acc = fmad(c7[3], lds[b0][sy - 1][sx], acc);
+      acc = fmad(c7[4], lds[b0][sy + 1][sx], acc);
+      acc = fmad(c7[5], pm1, acc);
+      acc = fmad(c7[6], pp1, acc);
       if (active) {
         q[(size_t)k * P + off] = acc;
         dot = fmad(p0, acc, dot);
@@ -670,6 +764,7 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
     p0 = pm1;
   };
   Stage s0, s1;
+  if (centre) zw_below = zwi[(size_t)nz * P + off];  // ZW(KE+1), the bottom face of the last level
   load_a(nz - 1, s1);
   load_a(nz - 2, s0);
   load_c(nz - 1, s0);
@@ -680,9 +775,7 @@ __global__ __launch_bounds__(NT_BWD) void k_bwd(const Sys* __restrict__ systems,
     if (k == 0) break;
     step(k - 1, s1, s0);
   }
-  T dummy = T(0);
-  block_sum2<T, NW_BWD>(dot, dummy);
-  publish2(s, t.local, dot, dummy);
+  publish_rows<T, H, H + 3>(s, t.bx, t.by, dot, T(0));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -701,7 +794,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const Sys* __restrict__ systems) 
   using Acc = typename std::conditional<(MODE == RED_NB || MODE == RED_RES), double, T>::type;
   gptr<const Acc> part = G<const Acc>(s.partials);
   Acc a = Acc(0), b = Acc(0);
-  for (int t = threadIdx.x; t < s.ntiles; t += NT) {
+  for (int t = threadIdx.x; t < s.nslots; t += NT) {
     a += part[2 * t];
     b += part[2 * t + 1];
   }
@@ -867,34 +960,73 @@ hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
   LAUNCH(k_refine<double>, ntiles, st, d_sys, d_tiles);
 }
 
-hipError_t launch_init(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys) {
-  if (fp32)
-    PHASE((k_fwd<float, true>), (launch_reduce<float, RED_INIT>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
-  PHASE((k_fwd<double, true>), (launch_reduce<double, RED_INIT>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
+template <int H>
+static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+  const dim3 grid(ntiles), block(64 * H);
+  if (fp32) {
+    if (init)
+      hipLaunchKernelGGL((k_fwd<float, true, H>), grid, block, 0, st, d_sys, d_tiles);
+    else
+      hipLaunchKernelGGL((k_fwd<float, false, H>), grid, block, 0, st, d_sys, d_tiles);
+  } else {
+    if (init)
+      hipLaunchKernelGGL((k_fwd<double, true, H>), grid, block, 0, st, d_sys, d_tiles);
+    else
+      hipLaunchKernelGGL((k_fwd<double, false, H>), grid, block, 0, st, d_sys, d_tiles);
+  }
+  return hipGetLastError();
 }
 
-hipError_t launch_update(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys) {
+template <int H>
+static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  const dim3 grid(ntiles), block(bwd_threads<H>());
   if (fp32)
-    PHASE((k_fwd<float, false>), (launch_reduce<float, RED_FWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
-  PHASE((k_fwd<double, false>), (launch_reduce<double, RED_FWD>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles);
-}
-
-hipError_t launch_spmv(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
-                       int raw) {
-  if (ntiles <= 0) return hipSuccess;
-  if (fp32)
-    hipLaunchKernelGGL(k_bwd<float>, dim3(ntiles), dim3(NT_BWD), 0, st, d_sys, d_tiles, raw);
+    hipLaunchKernelGGL((k_bwd<float, H>), grid, block, 0, st, d_sys, d_tiles, raw);
   else
-    hipLaunchKernelGGL(k_bwd<double>, dim3(ntiles), dim3(NT_BWD), 0, st, d_sys, d_tiles, raw);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return fp32 ? launch_reduce<float, RED_BWD>(st, d_sys, nsys) : launch_reduce<double, RED_BWD>(st, d_sys, nsys);
+    hipLaunchKernelGGL((k_bwd<double, H>), grid, block, 0, st, d_sys, d_tiles, raw);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+  if (ntiles <= 0) return hipSuccess;
+  switch (h) {
+    case 4: return fwd_h<4>(st, fp32, init, d_sys, d_tiles, ntiles);
+    case 8: return fwd_h<8>(st, fp32, init, d_sys, d_tiles, ntiles);
+    case 12: return fwd_h<12>(st, fp32, init, d_sys, d_tiles, ntiles);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  if (ntiles <= 0) return hipSuccess;
+  switch (h) {
+    case 4: return bwd_h<4>(st, fp32, d_sys, d_tiles, ntiles, raw);
+    case 8: return bwd_h<8>(st, fp32, d_sys, d_tiles, ntiles, raw);
+    case 12: return bwd_h<12>(st, fp32, d_sys, d_tiles, ntiles, raw);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_inner_reduce(hipStream_t st, bool fp32, InnerPhase ph, const Sys* d_sys, int nsys) {
+  switch (ph) {
+    case PH_INIT:
+      return fp32 ? launch_reduce<float, RED_INIT>(st, d_sys, nsys) : launch_reduce<double, RED_INIT>(st, d_sys, nsys);
+    case PH_BWD:
+      return fp32 ? launch_reduce<float, RED_BWD>(st, d_sys, nsys) : launch_reduce<double, RED_BWD>(st, d_sys, nsys);
+    case PH_FWD:
+      return fp32 ? launch_reduce<float, RED_FWD>(st, d_sys, nsys) : launch_reduce<double, RED_FWD>(st, d_sys, nsys);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_copy4(hipStream_t st, const void* src, void* dst, size_t bytes) {
   const size_t n4 = bytes / 16;
   hipLaunchKernelGGL(k_copy4, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n4);
   return hipGetLastError();
+}
+
+hipError_t launch_materialize(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, double* A) {
+  LAUNCH(k_materialize, ntiles, st, d_sys, d_tiles, A);
 }
 
 hipError_t launch_apply64(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* x,

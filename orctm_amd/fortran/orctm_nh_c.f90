@@ -28,12 +28,12 @@ MODULE orctm_nh_c
   TYPE, BIND(C) :: orctm_nh_options_t
     INTEGER(C_INT32_T) :: precision, maxit, max_outer, warm_start
     REAL(C_DOUBLE) :: rtol, rtol_inner
-    INTEGER(C_INT32_T) :: check_every, device, profile
-    INTEGER(C_INT32_T) :: reserved(5)
+    INTEGER(C_INT32_T) :: check_every, device, profile, tile_rows
+    INTEGER(C_INT32_T) :: reserved(4)
   END TYPE
 
   TYPE, BIND(C) :: orctm_nh_stats_t
-    INTEGER(C_INT32_T) :: status, outer_iters, inner_iters, reserved0
+    INTEGER(C_INT32_T) :: status, outer_iters, inner_iters, tile_rows
     REAL(C_DOUBLE) :: rel_res, bnorm, solve_ms, total_ms
     REAL(C_DOUBLE) :: reserved(4)
   END TYPE

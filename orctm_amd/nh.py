@@ -36,7 +36,8 @@ def _dp(a: np.ndarray):
 
 def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optional[float] = None,
             maxit: int = 10000, max_outer: int = 40, warm_start: bool = True, check_every: int = 16,
-            device: int = 0, profile: bool = False) -> abi.OptionsC:
+            device: int = 0, profile: bool = False, tile_rows: int = 0) -> abi.OptionsC:
+    """Solver options (orctm_nh_options_t). tile_rows: 0 = tuned per size class, or 4, 8, 12."""
     lib = abi.load()
     o = abi.OptionsC()
     lib.orctm_nh_default_options(ctypes.byref(o))
@@ -49,6 +50,7 @@ def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optiona
     o.check_every = check_every
     o.device = device
     o.profile = 1 if profile else 0
+    o.tile_rows = tile_rows
     return o
 
 
@@ -158,6 +160,11 @@ class NHSolver:
         self._rc(self.lib.orctm_nh_device_bytes(self.h, ctypes.byref(b)), "orctm_nh_device_bytes")
         return b.value
 
+    def state_bytes(self) -> int:
+        b = ctypes.c_int64()
+        self._rc(self.lib.orctm_nh_state_bytes(self.h, ctypes.byref(b)), "orctm_nh_state_bytes")
+        return b.value
+
     # -- helpers ---------------------------------------------------------------------------------
     def _check_f(self, a: np.ndarray, name: str) -> None:
         if a.dtype != np.float64 or a.shape != self.shape or not a.flags["F_CONTIGUOUS"]:
@@ -211,6 +218,74 @@ class NHBatch:
         if getattr(self, "b", None):
             self.lib.orctm_nh_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NHServer:
+    """Continuous-batching request loop (orctm_nh_server_*): requests = model solver + time steps."""
+
+    def __init__(self, hbm_budget: int = 0, max_active: int = 64, device: int = 0):
+        self.lib = abi.load()
+        o = abi.ServerOptionsC()
+        self.lib.orctm_nh_server_default_options(ctypes.byref(o))
+        o.hbm_budget = hbm_budget
+        o.max_active = max_active
+        o.device = device
+        srv = ctypes.c_void_p()
+        rc = self.lib.orctm_nh_server_create(ctypes.byref(o), ctypes.byref(srv))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_server_create")
+        self.s = srv
+        self._keep = {}
+
+    def submit(self, model: NHSolver, nsteps: int, perturb: float = 0.01, seed: int = 0,
+               out: Optional[np.ndarray] = None) -> int:
+        if out is not None:
+            model._check_f(out, "out")
+        t = ctypes.c_int64()
+        rc = self.lib.orctm_nh_server_submit(self.s, model.h, nsteps, perturb, seed,
+                                             _dp(out) if out is not None else None, ctypes.byref(t))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_server_submit")
+        self._keep[t.value] = (model, out)
+        return t.value
+
+    def poll(self, ticket: int):
+        st = abi.RequestStatsC()
+        rc = self.lib.orctm_nh_server_poll(self.s, ticket, ctypes.byref(st))
+        return rc == abi.OK, st.as_dict()
+
+    def wait(self, ticket: int) -> dict:
+        st = abi.RequestStatsC()
+        rc = self.lib.orctm_nh_server_wait(self.s, ticket, ctypes.byref(st))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_server_wait")
+        return st.as_dict()
+
+    def drain(self) -> None:
+        self.lib.orctm_nh_server_drain(self.s)
+
+    def info(self) -> dict:
+        r, b = ctypes.c_int64(), ctypes.c_int64()
+        a, q = ctypes.c_int32(), ctypes.c_int32()
+        self.lib.orctm_nh_server_info(self.s, ctypes.byref(r), ctypes.byref(b), ctypes.byref(a), ctypes.byref(q))
+        return {"rounds": r.value, "bytes_in_use": b.value, "active": a.value, "queued": q.value}
+
+    def close(self) -> None:
+        if getattr(self, "s", None):
+            self.lib.orctm_nh_server_destroy(self.s)
+            self.s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -73,7 +73,7 @@ def test_inner_operator_and_line_preconditioner_bit_exact(name, precision):
     g, divg, A, S, b = case(name)
     print(f"[inner] {name} precision={precision}", flush=True)
     dt = np.float32 if precision == abi.MIXED else np.float64
-    C = oracle.inner_coef(A, S, dt)
+    C = oracle.inner_coef(A, S, dt, grid=g)
     invD, L = oracle.line_factor(g, C)
     x = synth.normal(98, 2, g.cells).astype(dt)
     with NHSolver(g, options(precision=precision)) as s:
@@ -264,3 +264,57 @@ def test_perturbed_rhs_is_deterministic_and_keeps_land_dry():
     assert np.all(np.abs(bp[~wet]) <= 1e-12 * np.abs(b).max())
     rel = np.linalg.norm(bp - b) / np.linalg.norm(b)
     assert 0.002 < rel < 0.05
+
+
+@pytest.mark.parametrize("name", ["regional", "stretched_dx"])
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+def test_tile_height_does_not_change_results(name, precision):
+    """Block sums are formed over fixed 4-row groups, so every tile height gives the same bits."""
+    print(f"[tile] {name} precision={precision}", flush=True)
+    g, divg, A, S, b = case(name)
+    out = []
+    for rows in (4, 8, 12, 0):
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=precision, tile_rows=rows)) as s:
+            st = s.solve(divg, pnh)
+        assert st["status"] == "OK", st
+        assert st["tile_rows"] == rows or (rows == 0 and st["tile_rows"] in (4, 8, 12))
+        out.append((g.interior(pnh), st))
+    for p, st in out[1:]:
+        assert bits_equal(p, out[0][0])
+        assert st["inner_iters"] == out[0][1]["inner_iters"]
+        assert st["outer_iters"] == out[0][1]["outer_iters"]
+
+
+def test_batch_with_mixed_tile_heights_matches_alone():
+    names = ["regional", "cyclic_x", "one_layer"]
+    alone, solvers = {}, []
+    for n, rows in zip(names, (4, 8, 12)):
+        g, divg, A, S, b = case(n)
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=abi.MIXED, tile_rows=12)) as s:
+            st = s.solve(divg, pnh)
+        alone[n] = (g.interior(pnh), st)
+        s = NHSolver(g, options(precision=abi.MIXED, tile_rows=rows))
+        s.set_rhs(divg)
+        solvers.append(s)
+    batch = NHBatch(solvers)
+    rc, stats = batch.solve()
+    assert rc == abi.OK, stats
+    assert [st["tile_rows"] for st in stats] == [4, 8, 12]
+    for n, s, st in zip(names, solvers, stats):
+        g = s.grid
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        s.get_solution(pnh)
+        assert bits_equal(g.interior(pnh), alone[n][0]), n
+        assert st["inner_iters"] == alone[n][1]["inner_iters"]
+    batch.close()
+    for s in solvers:
+        s.close()
+
+
+def test_invalid_tile_rows_rejected():
+    g, divg, A, S, b = case("closed")
+    with pytest.raises(Exception) as ei:
+        NHSolver(g, options(precision=abi.MIXED, tile_rows=6))
+    assert getattr(ei.value, "code", None) == abi.ERR_INVALID

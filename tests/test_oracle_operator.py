@@ -356,3 +356,21 @@ def test_dense_transliteration_matches(name):
     Md = dense_reference(g)
     Mo = oracle.to_scipy(g, A).toarray()
     assert np.array_equal(Md, Mo)
+
+
+@pytest.mark.parametrize("name", list(SPECS) + list(TINY))
+def test_compact_form_rebuilds_the_operator_bit_for_bit(name):
+    """The device keeps ZW/ZP, 12 flag bits per cell and 7 metric products per column; rebuilding
+    every row from them with the assembly's operations must give compute_matrix's values exactly."""
+    g = build(name)[0] if name in SPECS else synth.make_grid(TINY[name])
+    A, S = oracle.assemble(g)
+    zw, zp, fl, col = oracle.compact(g)
+    A2 = oracle.rebuild64(g, zw, zp, fl, col)
+    assert np.array_equal(A.view(np.uint64), A2.view(np.uint64))
+    # the FP32 inner operator, evaluated in float from the float-rounded compact form, stays within a
+    # few float ulps of S*(-A) rounded once
+    C32 = oracle.inner_coef32(g)
+    ref = (S[None, :] * (-A)).astype(np.float32)
+    nz = ref != 0
+    assert np.array_equal(C32 != 0, nz)
+    assert np.max(np.abs(C32[nz] - ref[nz]) / np.abs(ref[nz])) < 1e-6
