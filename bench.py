@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--nsys", type=int, default=64, help="batch workload: number of systems")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
+    ap.add_argument("--tile-rows", type=int, default=0, help="inner-kernel tile height: 0 = tuned, or 4, 8, 12")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (systems would shard "
@@ -138,7 +139,7 @@ def main():
     results = {}
     sols = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        s = NHSolver(g, options(precision=prec, warm_start=False, profile=True))
+        s = NHSolver(g, options(precision=prec, warm_start=False, profile=True, tile_rows=args.tile_rows))
         s.set_rhs(divg)
         log(f"[bench] {name}: device bytes {s.device_bytes() / 1e9:.2f} GB")
         wall, lat, stats, prof = time_solves(s, args.steps, args.warmup)
@@ -189,6 +190,8 @@ def main():
         "inner_iters_mixed": inner,
         "outer_iters_mixed": mx["stats"][-1]["outer_iters"],
         "inner_iters_fp64": inner64,
+        "tile_rows_mixed": mx["stats"][-1]["tile_rows"],
+        "tile_rows_fp64": f64["stats"][-1]["tile_rows"],
         "rel_res_mixed": mx["stats"][-1]["rel_res"],
         "rel_res_fp64": f64["stats"][-1]["rel_res"],
         "rel_err_mixed_vs_fp64": err,
@@ -239,7 +242,7 @@ def bench_batch(args):
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         solvers = []
         for g, divg in systems:
-            s = NHSolver(g, options(precision=prec, warm_start=False, profile=True))
+            s = NHSolver(g, options(precision=prec, warm_start=False, profile=True, tile_rows=args.tile_rows))
             s.set_rhs(divg)
             solvers.append(s)
         batch = NHBatch(solvers)
@@ -269,7 +272,10 @@ def bench_batch(args):
             if r["rc"] != abi.OK:
                 raise RuntimeError(f"single solve failed: {r}")
         seq = time.perf_counter() - t1
-        out[name] = dict(wall=wall, lat=lat, inner=inner, cell_iters=cell_iters, mv_bytes=mv_bytes,
+        heights = {}
+        for x in st:
+            heights[x["tile_rows"]] = heights.get(x["tile_rows"], 0) + 1
+        out[name] = dict(wall=wall, lat=lat, inner=inner, cell_iters=cell_iters, mv_bytes=mv_bytes, heights=heights,
                          max_inner=max(x["inner_iters"] for x in st),
                          rel=max(x["rel_res"] for x in st), prof=prof, seq=seq)
         log(f"[bench] batch {name}: {len(systems) * args.steps / wall:.2f} solves/s, one-by-one "
@@ -294,6 +300,7 @@ def bench_batch(args):
         "one_by_one_solves_per_s_mixed": round(n / mx["seq"], 3),
         "cell_iters_per_s": round(mx["cell_iters"] * args.steps / mx["wall"], 1),
         "max_rel_res_mixed": mx["rel"], "max_rel_res_fp64": f64["rel"],
+        "tile_rows_mixed": mx["heights"], "tile_rows_fp64": f64["heights"],
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
