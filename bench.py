@@ -31,17 +31,21 @@ from orctm_amd.nh import NHSolver, options  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes of one launch, per cell and per (i, j) column, for word size W (4 FP32, 8 FP64).
-# k_bwd (back substitution + p update + 7-point matvec + p.q): reads w, l, p_old, ZW, ZP (5 words)
-#   and the uint16 coupling flags per cell, writes p and q (2 words); per column it reads the 7
-#   metric products and the bottom-face ZW (8 words).
-# k_fwd (x/r update + forward elimination + r.z, r.r): reads x, p, r, q, l, invD, writes x, r, w.
+# Algorithmic HBM bytes of one launch, per cell and per (i, j) column, for word size W (4 FP32, 8 FP64)
+# and line-preconditioner factor size F (2: fp16 in FP32 mode, 8 in FP64 mode).
+# k_bwd (back substitution + p update + 7-point matvec + p.q + the previous iteration's x += alpha p):
+#   reads w, p_old, x, ZW, ZP (5 words), l (F) and the uint16 coupling flags per cell, writes p, q, x
+#   (3 words); per column it reads the 7 metric products and the bottom-face ZW (8 words).
+# k_fwd (r update + forward elimination + r.z, r.r): reads r, q (2 words), l, invD (2 F), writes r, w;
+#   in FP32 mode also the per-column scale of invD (4 B per column).
 def bwd_bytes(cells, columns, W):
-    return cells * (7 * W + 2) + columns * 8 * W
+    F = 2 if W == 4 else 8
+    return cells * (8 * W + F + 2) + columns * 8 * W
 
 
-def fwd_bytes(cells, W):
-    return cells * 9 * W
+def fwd_bytes(cells, columns, W):
+    F = 2 if W == 4 else 8
+    return cells * (4 * W + 2 * F) + (columns * 4 if W == 4 else 0)
 
 
 def measured_traffic(name):
@@ -161,7 +165,9 @@ def main():
     prof = mx["prof"]
     mv_avg_ms = prof["matvec_ms"] / max(prof["matvec_launches"], 1)
     up_avg_ms = prof["update_ms"] / max(prof["update_launches"], 1)
-    mv_bytes = bwd_bytes(cells, columns, 4)
+    # the first iteration of each inner solve (one per refinement step) has no pending x update
+    first_frac = mx["stats"][-1]["outer_iters"] / max(mx["stats"][-1]["inner_iters"], 1)
+    mv_bytes = int(bwd_bytes(cells, columns, 4) - first_frac * 8 * cells)
     achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
     prof64 = f64["prof"]
     mv64_ms = prof64["matvec_ms"] / max(prof64["matvec_launches"], 1)
@@ -202,15 +208,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (back substitution + p update + 7-point matvec + p.q)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
-                     "bytes_per_launch": mv_bytes, "bytes_per_cell": 30, "bytes_per_column": 32,
+                     "bytes_per_launch": mv_bytes, "bytes_per_cell": 36, "bytes_per_column": 32,
                      "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
         "other_kernels": {
-            "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 36,
-                             "GBps": round(fwd_bytes(cells, 4) / (up_avg_ms * 1e-3) / 1e9, 1)},
-            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 58, "bytes_per_column": 64,
+            "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 20, "bytes_per_column": 4,
+                             "GBps": round(fwd_bytes(cells, columns, 4) / (up_avg_ms * 1e-3) / 1e9, 1)},
+            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 74, "bytes_per_column": 64,
                               "GBps": round(bwd_bytes(cells, columns, 8) / (mv64_ms * 1e-3) / 1e9, 1)},
-            "k_fwd<double>": {"avg_launch_ms": round(up64_ms, 5), "bytes_per_cell": 72,
-                              "GBps": round(fwd_bytes(cells, 8) / (up64_ms * 1e-3) / 1e9, 1)},
+            "k_fwd<double>": {"avg_launch_ms": round(up64_ms, 5), "bytes_per_cell": 48,
+                              "GBps": round(fwd_bytes(cells, columns, 8) / (up64_ms * 1e-3) / 1e9, 1)},
         },
         "copy_probe_GBps": round(abi.copy_probe(1 << 30, 20), 1),
         "cpu_baseline": None,
@@ -304,7 +310,7 @@ def bench_batch(args):
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "30 B per active cell-iteration + 32 B per active column-iteration over the summed "
+                     "note": "36 B per active cell-iteration + 32 B per active column-iteration over the summed "
                              "matvec launch time"},
         "cpu_baseline": None,
     }

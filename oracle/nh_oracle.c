@@ -330,6 +330,48 @@ int orc_scatter_interior(const orctm_nh_grid_t* g, const double* x, double* pnh)
 /* ------------------------------------------------------------------------------------------- */
 /* Precision-generic pieces: matvec, line preconditioner, PCG. Instantiated for double and float. */
 
+/*
+ * Storage of the FP32 line-preconditioner factors (device: nh_kernels.hip, k_line_factor<float>).
+ * Each value is kept as an IEEE fp16 (10-bit mantissa, round to nearest even), only in the normal
+ * fp16 range: l with |l| < 2^-14 is flushed to 0; invD is scaled per column by 2^sc, sc = 15 - ex
+ * where max_k |invD| = f 2^ex, f in [0.5, 1), so the column's largest scaled value lies in
+ * [2^14, 2^15); scaled magnitudes below 2^-14 are raised to 2^-14 (invD stays > 0, M stays SPD).
+ * half_round returns the fp16 value as a float (exactly representable); the scale 2^-sc is exact.
+ */
+static float half_round(float v) {
+  if (!(fabsf(v) < 65520.0f)) return isnan(v) ? v : copysignf(INFINITY, v);
+  if (v == 0.0f) return v;
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  u = (u + 0xFFFu + ((u >> 13) & 1u)) & 0xFFFFE000u; /* 23 -> 10 mantissa bits, ties to even */
+  memcpy(&v, &u, 4);
+  return v;
+}
+
+static void factor_round64(double* invD, double* L, size_t P, int nz, size_t col) {
+  (void)invD, (void)L, (void)P, (void)nz, (void)col; /* FP64 factors are kept as they are */
+}
+
+static void factor_round32(float* invD, float* L, size_t P, int nz, size_t col) {
+  const float lo = 6.103515625e-05f; /* 2^-14, the smallest normal fp16 */
+  float mx = 0.0f;
+  for (int k = 0; k < nz; ++k) {
+    const float a = fabsf(invD[col + P * (size_t)k]);
+    if (a > mx) mx = a;
+  }
+  int ex = 0;
+  if (mx > 0.0f && isfinite(mx)) frexpf(mx, &ex);
+  const int sc = (mx > 0.0f && isfinite(mx)) ? 15 - ex : 0;
+  for (int k = 0; k < nz; ++k) {
+    const size_t n = col + P * (size_t)k;
+    float v = ldexpf(invD[n], sc);
+    if (fabsf(v) < lo) v = copysignf(lo, v);
+    invD[n] = ldexpf(half_round(v), -sc);
+    const float l = L[n];
+    L[n] = fabsf(l) < lo ? 0.0f : half_round(l);
+  }
+}
+
 #define REAL double
 #define SFX(name) name##64
 #define FMA fma

@@ -114,6 +114,14 @@ namespace {
 
 thread_local char g_last_error[256] = "";
 
+// Graph capture on one thread and a null-stream, allocation or device-wide call on another do not
+// mix: the runtime fails the other thread's call with hipErrorStreamCaptureImplicit even in
+// thread-local capture mode. Every capture holds this lock, and so does every entry point that
+// allocates, frees, copies on the null stream or synchronises the device, so that handles can be
+// created and used from several threads (the serving loop runs its own thread).
+std::recursive_mutex g_dev_mu;
+#define DEVLOCK std::lock_guard<std::recursive_mutex> dev_lock_(g_dev_mu)
+
 hipError_t note_error(hipError_t e, const char* what, int line) {
   if (e != hipSuccess)
     snprintf(g_last_error, sizeof(g_last_error), "%s (%d) at nh_api.hip:%d: %s", hipGetErrorName(e), (int)e, line,
@@ -278,10 +286,10 @@ void order_into(std::vector<Tile>& nat, const Sys& s, int q, int h) {
 }
 
 // Times rounds of (matvec, column update) over the given systems at each candidate height and
-// returns the fastest. The kernels run against scratch inner states (first iteration, alpha = 0):
-// they overwrite the inner work vectors w, p, q and the partial sums, all of which every solve
-// re-initialises before use, and leave x and r as they are; the FP64 solution, the right-hand
-// side and the operator are not touched.
+// returns the fastest. The kernels run a full iteration's memory traffic against scratch inner
+// states (not the first iteration, alpha = beta = 0): they overwrite the inner work vectors w, p,
+// q and the partial sums, all of which every solve re-initialises before use, and store x and r
+// back unchanged; the FP64 solution, the right-hand side and the operator are not touched.
 hipError_t tune_height(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& members, int* best) {
   const int m = (int)members.size();
   std::vector<Sys> ts(m);
@@ -290,12 +298,10 @@ hipError_t tune_height(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
     if (c.fp32) {
       Inner<float>* in = reinterpret_cast<Inner<float>*>(hin.data()) + q;
       in->done = RUN;
-      in->first = 1;
       in->rho = 1.f;
     } else {
       Inner<double>* in = reinterpret_cast<Inner<double>*>(hin.data()) + q;
       in->done = RUN;
-      in->first = 1;
       in->rho = 1.0;
     }
   }
@@ -442,6 +448,7 @@ hipError_t ctx_chunk(Ctx& c) {
     return hipSuccess;
   }
   if (!c.graph) {
+    DEVLOCK;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
@@ -586,8 +593,11 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
     s.zp_in = s.zp;
     s.col_in = s.col;
   }
-  ALLOC(invD, NP * es, const void*);
-  ALLOC(L, NP * es, const void*);
+  // line-preconditioner factors: fp16 bits + a per-column scale for FP32, FP64 values otherwise
+  const size_t fs = h->fp32 ? sizeof(uint16_t) : sizeof(double);
+  ALLOC(invD, NP * fs, const void*);
+  ALLOC(L, NP * fs, const void*);
+  if (h->fp32) ALLOC(dscale, (size_t)h->plane * sizeof(float), const float*);
   ALLOC(smaxp, sizeof(unsigned long long), const double*);
 #undef ALLOC
   h->d_smax = reinterpret_cast<unsigned long long*>(const_cast<double*>(s.smaxp));
@@ -740,6 +750,7 @@ const char* orctm_nh_status_string(int s) {
 
 int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts, orctm_nh_t** handle) {
   if (!handle) return ORCTM_NH_ERR_INVALID;
+  DEVLOCK;
   *handle = nullptr;
   if (!grid_valid(grid)) return ORCTM_NH_ERR_INVALID;
   orctm_nh_options_t o;
@@ -785,6 +796,7 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
 
 int orctm_nh_destroy(orctm_nh_t* h) {
   if (!h) return ORCTM_NH_ERR_INVALID;
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   if (h->ctx.stream) hipStreamSynchronize(h->ctx.stream);
   ctx_destroy(h->ctx);
@@ -796,6 +808,7 @@ int orctm_nh_destroy(orctm_nh_t* h) {
 int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg) {
   if (!h || !divg) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
   hipError_t e = ensure_fort(h);
@@ -809,6 +822,7 @@ int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg) {
 int orctm_nh_set_guess(orctm_nh_t* h, const double* pnh) {
   if (!h || !pnh) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
   hipError_t e = ensure_fort(h);
@@ -837,6 +851,7 @@ int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
 int orctm_nh_get_solution(orctm_nh_t* h, double* pnh) {
   if (!h || !pnh) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   // interior only (mo_petsc.f90:223-227); the halo of pnh is left as it is
   for (int k = 0; k < h->nz; ++k) {
@@ -869,6 +884,7 @@ int orctm_nh_solve(orctm_nh_t* h, const double* divg, double* pnh, orctm_nh_stat
 int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
   if (!h || !x || !y) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t bytes = (size_t)h->plane * h->nz * sizeof(double);
   double *dx = nullptr, *dy = nullptr;
@@ -888,6 +904,7 @@ int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
 int orctm_nh_get_operator(orctm_nh_t* h, double* coef7) {
   if (!h || !coef7) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t NP = (size_t)h->plane * h->nz, N = (size_t)h->nx * h->ny * h->nz;
   double* dA = nullptr;
@@ -918,6 +935,7 @@ int orctm_nh_device_bytes(orctm_nh_t* h, int64_t* bytes) {
 int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
   if (!h || !x || !y) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
@@ -949,6 +967,7 @@ int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
 int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
   if (!h || !r || !z) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
@@ -986,6 +1005,7 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
   if (!src || !out) return ORCTM_NH_ERR_INVALID;
   *out = nullptr;
   std::lock_guard<std::mutex> lk(src->mu);
+  DEVLOCK;
   hipSetDevice(src->opts.device);
   orctm_nh_t* h = new (std::nothrow) orctm_nh_t();
   if (!h) return ORCTM_NH_ERR_NOMEM;
@@ -1026,6 +1046,7 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
 int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed) {
   if (!h) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
   hipSetDevice(h->opts.device);
   hipError_t e = launch_perturb(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, amplitude, seed);
   if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
@@ -1034,6 +1055,7 @@ int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed) {
 
 int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps) {
   if (bytes < 16 || reps <= 0 || !gbps) return ORCTM_NH_ERR_INVALID;
+  DEVLOCK;
   if (hipSetDevice(device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
   void *a = nullptr, *b = nullptr;
   hipStream_t st = nullptr;
@@ -1063,6 +1085,7 @@ int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbp
 
 int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_t** batch) {
   if (!batch || n <= 0 || !systems) return ORCTM_NH_ERR_INVALID;
+  DEVLOCK;
   *batch = nullptr;
   const bool fp32 = systems[0] ? systems[0]->fp32 : true;
   int max_outer = 0, check = 16, dev = 0;
@@ -1147,6 +1170,7 @@ int orctm_nh_batch_profile_read(orctm_nh_batch_t* b, orctm_nh_profile_t* p, int3
 
 int orctm_nh_batch_destroy(orctm_nh_batch_t* b) {
   if (!b) return ORCTM_NH_ERR_INVALID;
+  DEVLOCK;
   hipSetDevice(b->ctx.device);
   if (b->ctx.stream) hipStreamSynchronize(b->ctx.stream);
   ctx_destroy(b->ctx);

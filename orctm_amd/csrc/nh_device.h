@@ -43,7 +43,9 @@ struct Outer {
 template <typename T>
 struct Inner {
   T rho, pq, rr, rr0, alpha, beta, tol2, padT;
-  int iter, maxit, done, first, pbuf, pad[3];
+  int iter, maxit, done, first, pbuf;
+  int xpend;  // 1: x += alpha p of the last iteration still to be applied
+  int pad[2];
 };
 
 // One system as the kernels see it (a copy lives in device memory).
@@ -61,8 +63,9 @@ struct Sys {
   const void* zw_in;    // the same in the inner precision (FP32 copies for MIXED; aliases for FP64)
   const void* zp_in;
   const void* col_in;
-  const void* invD;     // line preconditioner: 1/D_k
-  const void* L;        // line preconditioner: l_k
+  const void* invD;     // line preconditioner 1/D_k: FP64, or fp16 bits scaled by dscale (FP32)
+  const void* L;        // line preconditioner l_k: FP64, or fp16 bits (FP32)
+  const float* dscale;  // FP32 only: per-column power of two that multiplies the stored invD
   double* b;            // right-hand side DIVG*WETO
   double* p64;          // solution (FP64)
   void* x;              // inner solution

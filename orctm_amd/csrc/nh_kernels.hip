@@ -37,6 +37,38 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Storage of the FP32 line-preconditioner factors: IEEE fp16 bits, normal range only (the rules of
+// the oracle's factor_round32): l with |l| < 2^-14 is stored as 0; invD is scaled by a power of two
+// per column (largest scaled value in [2^14, 2^15)) with magnitudes raised to at least 2^-14. The
+// rounding is done with integer operations (ties to even) so that it is the oracle's to the bit;
+// decoding is the exact fp16 -> fp32 conversion.
+__device__ __forceinline__ uint16_t half_bits(float v) {
+  const uint32_t u0 = __float_as_uint(v);
+  const uint32_t sgn = (u0 >> 16) & 0x8000u;
+  const float a = fabsf(v);
+  if (!(a < 65520.0f)) return isnan(v) ? (uint16_t)0x7E00u : (uint16_t)(sgn | 0x7C00u);
+  if (a == 0.0f) return (uint16_t)sgn;
+  const uint32_t u = (u0 + 0xFFFu + ((u0 >> 13) & 1u)) & 0xFFFFE000u;
+  const uint32_t e = ((u >> 23) & 0xFFu) - 112u;  // fp32 exponent bias 127 -> fp16 bias 15
+  return (uint16_t)(sgn | (e << 10) | ((u >> 13) & 0x3FFu));
+}
+
+__device__ __forceinline__ float half_val(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+constexpr float kHalfMin = 6.103515625e-05f;  // 2^-14
+
+// Factor storage per inner precision: fp16 bits for FP32, the value itself for FP64.
+template <typename T>
+struct Fac {
+  using S = T;
+  __device__ static T ld(gptr<const S> p, size_t i) { return p[i]; }
+};
+template <>
+struct Fac<float> {
+  using S = uint16_t;
+  __device__ static float ld(gptr<const uint16_t> p, size_t i) { return half_val(p[i]); }
+};
+
 // Sum of a and b over a block of NW wavefronts; valid in thread 0.
 template <typename T, int NW = NT / 64>
 __device__ __forceinline__ void block_sum2(T& a, T& b) {
@@ -291,26 +323,50 @@ __global__ __launch_bounds__(NT) void k_line_factor(const Sys* __restrict__ syst
   gptr<const T> zpi = G<const T>(s.zp_in);
   gptr<const T> cpi = G<const T>(s.col_in);
   gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
-  gptr<T> invD = G<T>(s.invD);
-  gptr<T> L = G<T>(s.L);
   T cc[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) cc[q] = cpi[q * P + col];
-  T Dprev = T(1), cd_prev = T(0);
-  for (int k = 0; k < s.nz; ++k) {
-    const size_t n = col + (size_t)k * P;
-    T c7[7];
-    inner_row<T>(cc, zwi[n], zwi[n + P], zpi[n], flg[n], c7);
-    T d = c7[0], l = T(0);
-    if (k > 0) {
-      const T e = (cd_prev != T(0) && c7[5] != T(0)) ? c7[5] : T(0);
-      l = e / Dprev;
-      d = fmad(-l, e, d);
+  // the recurrence of the factorisation; emit(k, invD_k, l_k) receives each level's factors
+  auto factor = [&](auto&& emit) {
+    T Dprev = T(1), cd_prev = T(0);
+    for (int k = 0; k < s.nz; ++k) {
+      const size_t n = col + (size_t)k * P;
+      T c7[7];
+      inner_row<T>(cc, zwi[n], zwi[n + P], zpi[n], flg[n], c7);
+      T d = c7[0], l = T(0);
+      if (k > 0) {
+        const T e = (cd_prev != T(0) && c7[5] != T(0)) ? c7[5] : T(0);
+        l = e / Dprev;
+        d = fmad(-l, e, d);
+      }
+      emit(n, T(1) / d, l);
+      Dprev = d;
+      cd_prev = c7[6];
     }
-    invD[n] = T(1) / d;
-    L[n] = l;
-    Dprev = d;
-    cd_prev = c7[6];
+  };
+  if constexpr (std::is_same<T, float>::value) {
+    float mx = 0.0f;
+    factor([&](size_t, float inv, float) { mx = fmaxf(mx, fabsf(inv)); });
+    int ex = 0;
+    const bool ok = mx > 0.0f && isfinite(mx);
+    if (ok) frexpf(mx, &ex);
+    const int sc = ok ? 15 - ex : 0;
+    G<float>(s.dscale)[col] = ldexpf(1.0f, -sc);
+    gptr<uint16_t> invD = G<uint16_t>(s.invD);
+    gptr<uint16_t> L = G<uint16_t>(s.L);
+    factor([&](size_t n, float inv, float l) {
+      float v = ldexpf(inv, sc);
+      if (fabsf(v) < kHalfMin) v = copysignf(kHalfMin, v);
+      invD[n] = half_bits(v);
+      L[n] = fabsf(l) < kHalfMin ? (uint16_t)0 : half_bits(l);
+    });
+  } else {
+    gptr<T> invD = G<T>(s.invD);
+    gptr<T> L = G<T>(s.L);
+    factor([&](size_t n, T inv, T l) {
+      invD[n] = inv;
+      L[n] = l;
+    });
   }
 }
 
@@ -522,26 +578,34 @@ __global__ __launch_bounds__(NT) void k_refine(const Sys* __restrict__ systems, 
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   if (i >= s.nx || j >= s.ny) return;
   const double g = o->inv_gamma;
+  const Inner<T>* in = reinterpret_cast<const Inner<T>*>(s.inner);
+  // the inner solve's last x += alpha p, left pending by its last column update (k_bwd applies
+  // the others); p of that iteration is the buffer the next matvec would have read as p_old
+  const bool pend = in->xpend != 0;
+  const T alpha = in->alpha;
   gptr<const T> x = G<const T>(s.x);
+  gptr<const T> pl = G<const T>(in->pbuf ? s.pb : s.pa);
   gptr<double> p64 = G<double>(s.p64);
   const size_t col = (size_t)j * s.pitch + i;
 #pragma unroll 4
   for (int k = 0; k < s.nz; ++k) {
     const size_t idx = col + (size_t)k * s.plane;
-    p64[idx] = p64[idx] + (double)x[idx] * g;
+    const T xk = pend ? fmad(alpha, pl[idx], x[idx]) : x[idx];
+    p64[idx] = p64[idx] + (double)xk * g;
   }
 }
 
 // ------------------------------------------------------------------------------------------------
 // Inner PCG, two kernels per iteration.
 //
-//   k_fwd  (per column, top-down):   x += alpha p; r -= alpha q; forward elimination of the line
-//          preconditioner  y_k = r_k - l_k y_{k-1},  w_k = y_k / D_k;  r.z = sum y_k w_k,  r.r.
-//          Reads x p r q l invD, writes x r w: 9 words per cell.
+//   k_fwd  (per column, top-down):   r -= alpha q; forward elimination of the line preconditioner
+//          y_k = r_k - l_k y_{k-1},  w_k = y_k / D_k;  r.z = sum y_k w_k,  r.r.
+//          Reads r q l invD, writes r w: 4 words + 2 factors per cell (factors fp16 in FP32 mode).
 //   k_bwd  (64 x H tile, bottom-up): back substitution z_k = w_k - l_{k+1} z_{k+1} fused with
-//          p = z + beta p_old, q = C p (7-point, horizontal neighbours through LDS) and p.q; the
-//          rows of C are rebuilt from ZW, ZP, the flag bits and the column's metric products.
-//          Reads w l p_old ZW ZP (+ 2 B of flags), writes p q: 7 words + 2 B per cell.
+//          p = z + beta p_old, q = C p (7-point, horizontal neighbours through LDS) and p.q, and
+//          the previous iteration's x += alpha p_old; the rows of C are rebuilt from ZW, ZP, the
+//          flag bits and the column's metric products.
+//          Reads w l p_old x ZW ZP (+ 2 B of flags), writes p q x: 8 words + factor + 2 B per cell.
 // The halo lanes of k_bwd run the back-substitution recurrence of their halo columns themselves, so
 // z never goes to memory and no grid-wide step separates the two halves of the line solve.
 
@@ -563,18 +627,20 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
   gptr<T> r = G<T>(s.r);
   gptr<T> w = G<T>(s.z);
   gptr<const T> q = G<const T>(s.q);
-  gptr<const T> p = G<const T>(in->pbuf ? s.pa : s.pb);
-  gptr<const T> L = G<const T>(s.L);
-  gptr<const T> invD = G<const T>(s.invD);
+  gptr<const typename Fac<T>::S> L = G<const typename Fac<T>::S>(s.L);
+  gptr<const typename Fac<T>::S> invD = G<const typename Fac<T>::S>(s.invD);
   const T alpha = INIT ? T(0) : in->alpha;
   T rho = T(0), rr = T(0);
   if (i < s.nx && j < s.ny) {
+    // FP32 factors: invD is stored scaled by a power of two per column; undoing it is exact
+    T cs = T(1);
+    if constexpr (std::is_same<T, float>::value) cs = G<const float>(s.dscale)[col];
     // Batches of U levels in two register sets (ping-pong): the loads of batch b+1 are issued before
     // batch b is computed and stored, so stores (which may alias later loads as far as the compiler
     // knows) never sit between a load and its use, and no prefetched value has to be moved.
     constexpr int U = 4;
     struct Batch {
-      T x[U], p[U], r[U], q[U], l[U], d[U];
+      T r[U], q[U], l[U], d[U];
     };
     Batch b0, b1;
     auto load = [&](int k0, Batch& b) {
@@ -583,13 +649,9 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
         const int k = k0 + u < nz ? k0 + u : nz - 1;
         const size_t idx = col + (size_t)k * P;
         b.r[u] = r[idx];
-        b.l[u] = L[idx];
-        b.d[u] = invD[idx];
-        if (!INIT) {
-          b.x[u] = x[idx];
-          b.p[u] = p[idx];
-          b.q[u] = q[idx];
-        }
+        b.l[u] = Fac<T>::ld(L, idx);
+        b.d[u] = Fac<T>::ld(invD, idx) * cs;
+        if (!INIT) b.q[u] = q[idx];
       }
     };
     T yprev = T(0);
@@ -602,7 +664,6 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
           if (INIT) {
             x[idx] = T(0);
           } else {
-            x[idx] = fmad(alpha, b.p[u], b.x[u]);
             rk = fmad(-alpha, b.q[u], rk);
             r[idx] = rk;
           }
@@ -652,8 +713,13 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   const bool first = in->first != 0;
   const int pb = in->pbuf;
   const T beta = in->beta;
+  // the previous iteration's solution update x += alpha p_old, deferred to here where p_old is read
+  // anyway (not on an inner solve's first iteration, not in the verification mode)
+  const bool xupd = !first && !raw;
+  const T alpha = in->alpha;
   gptr<const T> w = G<const T>(s.z);
-  gptr<const T> L = G<const T>(s.L);
+  gptr<const typename Fac<T>::S> L = G<const typename Fac<T>::S>(s.L);
+  gptr<T> xv = G<T>(s.x);
   gptr<const T> pold = G<const T>(pb ? s.pb : s.pa);
   gptr<T> pnew = G<T>(pb ? s.pa : s.pb);
   gptr<T> q = G<T>(s.q);
@@ -700,7 +766,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   // data), set `nxt` receives w/l/p_old of level k-2 and the row data (ZW, ZP, flags) of level k-1.
   // The 7 inner coefficients are rebuilt from the row data and this column's metric products.
   struct Stage {
-    T w, l, p;
+    T w, l, p, x;
     T zw, zp;
     unsigned f;
   };
@@ -716,8 +782,9 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   auto load_a = [&](int m, Stage& st) {
     const size_t om = (size_t)(m > 0 ? m : 0) * P + off;
     st.w = w[om];
-    st.l = L[om];
+    st.l = Fac<T>::ld(L, om);
     st.p = pold[om];
+    if (centre && xupd) st.x = xv[om];
   };
   auto load_c = [&](int m, Stage& st) {
     if (!centre) return;
@@ -734,7 +801,10 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     lc = raw ? T(0) : st.l;
     const T pout = val ? pm : T(0);
     if (slot) lds[m % 3][sy][sx] = pout;
-    if (active) pnew[(size_t)m * P + off] = pm;
+    if (active) {
+      pnew[(size_t)m * P + off] = pm;
+      if (xupd) xv[(size_t)m * P + off] = fmad(alpha, st.p, st.x);
+    }
     return pout;
   };
   T p0 = T(0), pp1 = T(0), dot = T(0);
@@ -841,6 +911,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const Sys* __restrict__ systems) 
     in->iter = 0;
     in->first = 1;
     in->pbuf = 0;
+    in->xpend = 0;
     in->beta = T(0);
     if (rr == T(0))
       in->done = CONVERGED;
@@ -867,12 +938,14 @@ __global__ __launch_bounds__(NT) void k_reduce(const Sys* __restrict__ systems) 
     }
     in->first = 0;
     in->pbuf ^= 1;
+    in->xpend = 1;  // x += alpha p of this iteration: applied by the next matvec or by k_refine
     in->done = d;
   } else {  // RED_BWD
     const T pq = a;
     const T alpha = in->rho / pq;
     in->pq = pq;
     in->alpha = alpha;
+    in->xpend = 0;  // the matvec just applied the previous iteration's update
     if (!(pq > T(0)) || !isfinite((double)alpha)) {
       in->done = BREAKDOWN;
       if (in->iter == 0) o->odone = BREAKDOWN;

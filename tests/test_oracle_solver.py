@@ -26,31 +26,84 @@ def case(name):
     return _cache[name]
 
 
+def _tridiagonal(C64, idx):
+    """Per-column tridiagonal of the line preconditioner: diagonal C_centre and the symmetric
+    off-diagonal C_up(k) where both C_down(k-1) and C_up(k) are non-zero (wet-wet pairs)."""
+    d = C64[0, idx]
+    e = np.where((C64[6, idx[:-1]] != 0) & (C64[5, idx[1:]] != 0), C64[5, idx[1:]], 0.0)
+    return d, e
+
+
+def _banded(d, e):
+    ab = np.zeros((3, len(d)))
+    ab[0, 1:] = e
+    ab[1, :] = d
+    ab[2, :-1] = e
+    return ab
+
+
+def _columns(P):
+    return list(range(0, P, max(1, P // 25))) + [P - 1]
+
+
 @pytest.mark.parametrize("name", ["regional", "transect", "stretched_dx", "one_layer"])
-@pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_line_preconditioner_matches_banded_solve(name, dt):
+def test_line_preconditioner_matches_banded_solve(name):
     g, divg, A, S, b = case(name)
-    Cm = oracle.inner_coef(A, S, dt)
+    Cm = oracle.inner_coef(A, S, np.float64)
     invD, L = oracle.line_factor(g, Cm)
-    r = synth.normal(5, 9, g.cells).astype(dt)
+    r = synth.normal(5, 9, g.cells)
     z, rho = oracle.line_apply(g, invD, L, r)
     P = g.nx * g.ny
-    nz = g.nz
+    for col in _columns(P):
+        idx = col + P * np.arange(g.nz)
+        zr = sla.solve_banded((1, 1), _banded(*_tridiagonal(Cm, idx)), r[idx])
+        assert np.allclose(z[idx], zr, rtol=1e-12, atol=1e-12 * np.abs(zr).max())
+    rho_ref = float(np.dot(r, z))
+    assert abs(rho - rho_ref) <= 1e-12 * abs(rho_ref)
+
+
+@pytest.mark.parametrize("name", ["regional", "transect", "stretched_dx", "one_layer"])
+def test_fp32_line_factors_are_fp16_rounded_ldlt(name):
+    """The FP32 factors are the LDL^T factors of the FP32 tridiagonal stored as fp16 (l flushed to 0
+    below 2^-14; invD scaled by a power of two per column, largest scaled value in [2^14, 2^15)).
+    Applying them is an exact solve with the SPD tridiagonal M' = L'D'L'^T they define, and M'
+    is within fp16 rounding of the true tridiagonal M."""
+    g, divg, A, S, b = case(name)
+    Cm = oracle.inner_coef(A, S, np.float32)
+    invD, L = oracle.line_factor(g, Cm)
+    P, nz = g.nx * g.ny, g.nz
+    # storage rules
+    Lc = L.astype(np.float64)
+    assert np.all(np.float32(np.float16(L)) == L)
+    assert np.all((Lc == 0) | (np.abs(Lc) >= 2.0 ** -14))
+    iv = invD.astype(np.float64).reshape(nz, P)
+    mx = np.abs(iv).max(axis=0)
+    sc = 15 - np.frexp(mx)[1]
+    scaled = iv * np.exp2(sc)[None, :]
+    assert np.all(np.float16(scaled).astype(np.float64) == scaled)
+    assert np.all((np.abs(scaled).max(axis=0) >= 2.0 ** 14) & (np.abs(scaled).max(axis=0) < 2.0 ** 15))
+    assert np.all(iv > 0)
+    # the exact FP64 factorisation of the same FP32 tridiagonal, for the closeness checks
     C64 = Cm.astype(np.float64)
-    tol = 1e-12 if dt == np.float64 else 2e-4
-    for col in list(range(0, P, max(1, P // 25))) + [P - 1]:
+    invD64, L64 = oracle.line_factor(g, C64)
+    assert np.all(np.abs(iv.ravel() - invD64) <= 2.0 ** -10 * np.abs(invD64))
+    r = synth.normal(5, 9, g.cells).astype(np.float32)
+    z, rho = oracle.line_apply(g, invD, L, r)
+    for col in _columns(P):
         idx = col + P * np.arange(nz)
-        d = C64[0, idx]
-        # symmetric off-diagonal between k-1 and k: C_up(k) where both C_down(k-1) and C_up(k) are non-zero
-        e = np.where((C64[6, idx[:-1]] != 0) & (C64[5, idx[1:]] != 0), C64[5, idx[1:]], 0.0)
-        ab = np.zeros((3, nz))
-        ab[0, 1:] = e
-        ab[1, :] = d
-        ab[2, :-1] = e
-        zr = sla.solve_banded((1, 1), ab, r[idx].astype(np.float64))
-        assert np.allclose(z[idx], zr, rtol=tol, atol=tol * np.abs(zr).max())
+        d, e = _tridiagonal(C64, idx)
+        # M' from the stored factors: D'_k = 1/invD'_k, M'_kk = D'_k + l'_k^2 D'_{k-1}, M'_{k-1,k} = l'_k D'_{k-1}
+        Dp = 1.0 / invD[idx].astype(np.float64)
+        lp = L[idx].astype(np.float64)
+        dm = Dp.copy()
+        dm[1:] += lp[1:] ** 2 * Dp[:-1]
+        em = lp[1:] * Dp[:-1]
+        scale = np.abs(d).max()
+        assert np.all(np.abs(dm - d) <= 3e-3 * scale) and np.all(np.abs(em - e) <= 3e-3 * scale)
+        zr = sla.solve_banded((1, 1), _banded(dm, em), r[idx].astype(np.float64))
+        assert np.allclose(z[idx], zr, rtol=2e-4, atol=2e-4 * np.abs(zr).max())
     rho_ref = float(np.dot(r.astype(np.float64), z.astype(np.float64)))
-    assert abs(rho - rho_ref) <= (1e-12 if dt == np.float64 else 1e-4) * abs(rho_ref)
+    assert abs(rho - rho_ref) <= 1e-4 * abs(rho_ref)
 
 
 @pytest.mark.parametrize("name", list(SPECS))
