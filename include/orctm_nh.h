@@ -194,6 +194,48 @@ int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* prof, int32_t reset
 int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
 
 /* ------------------------------------------------------------------------------------------
+ * The steps of OCVTOT_NH on either side of the solve (ORCTM/src_orctm/ocvtot_nh.f90), on the
+ * device, for one process holding the whole domain. All arrays are the caller's Fortran arrays with
+ * halos (mo_commo1.f90:105-146); sizes, I_start/J_start and the cyclic/open flags are those of the
+ * grid the handle was created with. Expressions are evaluated left to right as the Fortran source
+ * writes them (FP64). */
+typedef struct orctm_nh_vgrid {
+  const double* dlxp;    /* DLXP(IE,JE) */
+  const double* dlyp;    /* DLYP(IE,JE) */
+  const double* ddpo;    /* DDPO(IE,JE,KE) */
+  const double* weto;    /* WETO(IE,JE,KE) */
+  const double* amsuo;   /* AMSUO(I_start:IE,JE,KE) */
+  const double* dduo;    /* DDUO(I_start:IE,JE,KE) */
+  const double* dtdxuo;  /* DTDXUO(I_start:IE,JE) = DT/DLXU (beleg.f90:66) */
+  const double* dlyu;    /* DLYU(I_start:IE,JE) */
+  const double* amsue;   /* AMSUE(IE,J_start:JE,KE) */
+  const double* ddue;    /* DDUE(IE,J_start:JE,KE) */
+  const double* dpye;    /* DPYE(IE,J_start:JE) = DT/DLYV (beleg.f90:74) */
+  const double* dlxv;    /* DLXV(IE,J_start:JE) */
+  const double* dtdxpo;  /* DTDXPO(IE,JE) = DT/DLXP (beleg.f90:48) */
+  const double* dtdyo;   /* DTDYO(IE,JE) = DT/DLYP (beleg.f90:49) */
+  double dt;             /* DT */
+  double dti;            /* DTI = 1/DT */
+} orctm_nh_vgrid_t;
+/* Uploads the velocity-side metrics once (they are constant over a run, like compute_matrix's). */
+int orctm_nh_attach_vgrid(orctm_nh_t* h, const orctm_nh_vgrid_t* vg);
+/* ocvtot_nh.f90:11-62 + the right-hand side of petsc_solve_3D: GU = (UOO-UOO_1)/DT, GV, GW likewise,
+ * DIVG = DIVGU + DIVGV + DIVGW on the interior cells, then b = DIVG*WETO resident on the device
+ * (ComputeRHS, mo_petsc.f90:376), as orctm_nh_set_rhs(DIVG) would. UOO, UOO_1 (I_start:IE,JE,KE);
+ * VOE, VOE_1 (IE,J_start:JE,KE); WNO, WNO_1 (IE,JE,KEP); ZO (IE,JE). If divg is not NULL its
+ * interior receives DIVG (the halo, which only the exchange of :43-57 would fill, is left as is). */
+int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, const double* voe,
+                        const double* voe_1, const double* wno, const double* wno_1, const double* zo,
+                        double* divg);
+/* ocvtot_nh.f90:151-230 after a solve: PNH gets the resident solution in its interior, then
+ * bounds_exch(PNH) (one process: cyclic copies only, mo_parallel.f90:551-668) and the open-boundary
+ * copies (:154-186) fill its halo (other halo cells keep the caller's values); UOO and VOE receive the
+ * pressure correction (:189-201) and bounds_exch; WNO is recomputed from continuity (:207-228) and
+ * exchanged. pnh, uoo, voe are read and written, wno (IE,JE,KEP) is written. The copies UOO_1 = UOO,
+ * VOE_1 = VOE, WNO_1 = WNO, WO = WNO (:232-249) stay with the caller. */
+int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, double* wno);
+
+/* ------------------------------------------------------------------------------------------
  * Building blocks of the inner solve, exposed for verification. No counterpart in the reference
  * (its Krylov internals live inside PETSc). "inner" is the scaled operator C = S (-A),
  * S = DLXP*DLYP/ZP per cell, held in FP32 (MIXED) or FP64 (FP64); x, y, r, z are interior vectors

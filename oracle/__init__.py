@@ -236,3 +236,64 @@ def to_scipy(grid, A):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+# ---- OCVTOT_NH around the solve (nh_oracle_ocv.c) ------------------------------------------------
+
+
+class _VGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("IE", "JE", "KE", "i_start", "j_start", "cyc_x", "cyc_y", "open_w",
+                                            "open_e", "open_n", "open_s")] + \
+               [(n, _dp) for n in ("dlxp", "dlyp", "ddpo", "weto", "amsuo", "dduo", "dtdxuo", "dlyu", "amsue",
+                                   "ddue", "dpye", "dlxv", "dtdxpo", "dtdyo")] + \
+               [("dt", ctypes.c_double), ("dti", ctypes.c_double)]
+
+
+def _vgrid(grid, v):
+    s = grid.spec
+    g = _VGrid(v.IE, v.JE, v.KE, v.i_start, v.j_start, int(s.cyclic_x), int(s.cyclic_y), int(s.open_w),
+               int(s.open_e), int(s.open_n), int(s.open_s))
+    for n in ("dlxp", "dlyp", "ddpo", "weto", "amsuo", "dduo", "dtdxuo", "dlyu", "amsue", "ddue", "dpye", "dlxv",
+              "dtdxpo", "dtdyo"):
+        a = getattr(v, n)
+        assert a.flags.f_contiguous and a.dtype == np.float64
+        setattr(g, n, a.ctypes.data_as(_dp))
+    g.dt, g.dti = v.dt, v.dti
+    g._keep = v
+    return g
+
+
+def _ocv_lib():
+    L = lib()
+    if not getattr(L, "_ocv", False):
+        L.orc_divergence.argtypes = [ctypes.POINTER(_VGrid)] + [_dp] * 8
+        L.orc_project.argtypes = [ctypes.POINTER(_VGrid)] + [_dp] * 4
+        L.orc_bounds_exch.argtypes = [_dp] + [ctypes.c_int] * 7
+        L._ocv = True
+    return L
+
+
+def divergence(grid, v, uoo=None, voe=None, wno=None, divg=None):
+    """DIVG (IE,JE,KE) of ocvtot_nh.f90:11-62 on the interior cells (halo of `divg` kept, zero if new)."""
+    L = _ocv_lib()
+    g = _vgrid(grid, v)
+    uoo = v.uoo if uoo is None else uoo
+    voe = v.voe if voe is None else voe
+    wno = v.wno if wno is None else wno
+    out = np.zeros((v.IE, v.JE, v.KE), order="F") if divg is None else divg
+    L.orc_divergence(ctypes.byref(g), _d(uoo), _d(v.uoo_1), _d(voe), _d(v.voe_1), _d(wno), _d(v.wno_1), _d(v.zo),
+                     _d(out))
+    return out
+
+
+def project(grid, v, pnh, uoo=None, voe=None):
+    """ocvtot_nh.f90:151-236 in place on copies: returns (pnh, uoo, voe, wno) after the halo exchange and
+    open-boundary copies of PNH, the U/V pressure correction and the continuity integration of WNO."""
+    L = _ocv_lib()
+    g = _vgrid(grid, v)
+    p = np.array(pnh, order="F", dtype=np.float64, copy=True)
+    u = np.array(v.uoo if uoo is None else uoo, order="F", dtype=np.float64, copy=True)
+    w_ = np.array(v.voe if voe is None else voe, order="F", dtype=np.float64, copy=True)
+    wn = np.zeros((v.IE, v.JE, v.KE + 1), order="F")
+    L.orc_project(ctypes.byref(g), _d(p), _d(u), _d(w_), _d(wn))
+    return p, u, w_, wn

@@ -81,6 +81,13 @@ class RequestStatsC(ctypes.Structure):
 
 
 # Every symbol include/orctm_nh.h declares (checked by the CPU test suite).
+class VGridC(ctypes.Structure):
+    """orctm_nh_vgrid_t: velocity-side metrics of OCVTOT_NH (Fortran arrays with halos) and DT, DTI."""
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in
+                ("dlxp", "dlyp", "ddpo", "weto", "amsuo", "dduo", "dtdxuo", "dlyu", "amsue", "ddue", "dpye",
+                 "dlxv", "dtdxpo", "dtdyo")] + [("dt", ctypes.c_double), ("dti", ctypes.c_double)]
+
+
 EXPORTS = [
     "orctm_nh_abi_version", "orctm_nh_device_count", "orctm_nh_default_options", "orctm_nh_status_string",
     "orctm_nh_last_error",
@@ -91,7 +98,7 @@ EXPORTS = [
     "orctm_nh_batch_profile_read", "orctm_nh_clone", "orctm_nh_perturb_rhs", "orctm_nh_copy_probe",
     "orctm_nh_state_bytes", "orctm_nh_server_default_options", "orctm_nh_server_create", "orctm_nh_server_submit",
     "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
-    "orctm_nh_server_destroy",
+    "orctm_nh_server_destroy", "orctm_nh_attach_vgrid", "orctm_nh_divergence", "orctm_nh_project",
 ]
 
 _lib = None

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -21,6 +22,7 @@
 
 #include "../../include/orctm_nh.h"
 #include "nh_kernels.h"
+#include "nh_ocv.h"
 
 using namespace onh;
 
@@ -102,6 +104,12 @@ struct orctm_nh {
   double* d_nb = nullptr;          // ||b|| written by k_rhs
   unsigned long long* d_smax = nullptr;
   Ctx ctx;                         // single-system context
+  // grid flags kept for the steps around the solve (orctm_nh_divergence / orctm_nh_project)
+  int i_start = 1, j_start = 1, cyc_x = 0, cyc_y = 0, open_w = 0, open_e = 0, open_n = 0, open_s = 0;
+  bool ocv_ready = false;
+  OcvArrays ocv{};                 // metrics attached by orctm_nh_attach_vgrid + velocity buffers
+  double *d_uoo = nullptr, *d_uoo_1 = nullptr, *d_voe = nullptr, *d_voe_1 = nullptr;
+  double *d_wno = nullptr, *d_wno_1 = nullptr, *d_zo = nullptr;
   std::mutex mu;
 };
 
@@ -777,6 +785,14 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   h->IE = grid->ie;
   h->JE = grid->je;
   h->KE = grid->ke;
+  h->i_start = grid->i_start;
+  h->j_start = grid->j_start;
+  h->cyc_x = grid->cyclic_x ? 1 : 0;
+  h->cyc_y = grid->cyclic_y ? 1 : 0;
+  h->open_w = grid->open_w ? 1 : 0;
+  h->open_e = grid->open_e ? 1 : 0;
+  h->open_n = grid->open_n ? 1 : 0;
+  h->open_s = grid->open_s ? 1 : 0;
   h->nx = grid->ie - 2;
   h->ny = grid->je - 2;
   h->nz = grid->ke;
@@ -1020,6 +1036,14 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
   h->pitch = src->pitch;
   h->plane = src->plane;
   h->fp32 = src->fp32;
+  h->i_start = src->i_start;
+  h->j_start = src->j_start;
+  h->cyc_x = src->cyc_x;
+  h->cyc_y = src->cyc_y;
+  h->open_w = src->open_w;
+  h->open_e = src->open_e;
+  h->open_n = src->open_n;
+  h->open_s = src->open_s;
   h->sys = src->sys;  // operator pointers shared; state pointers replaced below
   h->d_smax = src->d_smax;
   int rc = alloc_state(h);
@@ -1080,6 +1104,136 @@ int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbp
   if (st) hipStreamDestroy(st);
   hipFree(a);
   hipFree(b);
+  return map_err(e);
+}
+
+int orctm_nh_attach_vgrid(orctm_nh_t* h, const orctm_nh_vgrid_t* vg) {
+  if (!h || !vg) return ORCTM_NH_ERR_INVALID;
+  const double* in2[] = {vg->dlxp, vg->dlyp, vg->ddpo, vg->weto, vg->amsuo, vg->dduo, vg->dtdxuo,
+                         vg->dlyu, vg->amsue, vg->ddue, vg->dpye, vg->dlxv, vg->dtdxpo, vg->dtdyo};
+  for (const double* q : in2)
+    if (!q) return ORCTM_NH_ERR_INVALID;
+  if (!(vg->dt > 0.0) || !std::isfinite(vg->dti)) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
+  hipSetDevice(h->opts.device);
+  if (h->ocv_ready) return ORCTM_NH_ERR_INVALID;  // attached once per handle
+  const size_t IE = h->IE, JE = h->JE, KE = h->KE;
+  const size_t NU = IE - h->i_start + 1, NV = JE - h->j_start + 1;
+  const size_t p2 = IE * JE, u2 = NU * JE, v2 = IE * NV;
+  OcvArrays& a = h->ocv;
+  a = OcvArrays{};
+  a.IE = (int)IE;
+  a.JE = (int)JE;
+  a.KE = (int)KE;
+  a.i_start = h->i_start;
+  a.j_start = h->j_start;
+  a.cyc_x = h->cyc_x;
+  a.cyc_y = h->cyc_y;
+  a.open_w = h->open_w;
+  a.open_e = h->open_e;
+  a.open_n = h->open_n;
+  a.open_s = h->open_s;
+  a.dt = vg->dt;
+  a.dti = vg->dti;
+  hipError_t e = hipSuccess;
+  auto put = [&](const double*& dst, const double* src, size_t n) {
+    void* p = nullptr;
+    if (e == hipSuccess) e = dalloc(h, &p, n * sizeof(double));
+    if (e == hipSuccess) e = upload(p, src, n * sizeof(double));
+    dst = static_cast<const double*>(p);
+  };
+  auto buf = [&](double*& dst, size_t n) {
+    void* p = nullptr;
+    if (e == hipSuccess) e = dalloc(h, &p, n * sizeof(double));
+    dst = static_cast<double*>(p);
+  };
+  put(a.dlxp, vg->dlxp, p2);
+  put(a.dlyp, vg->dlyp, p2);
+  put(a.dtdxpo, vg->dtdxpo, p2);
+  put(a.dtdyo, vg->dtdyo, p2);
+  put(a.ddpo, vg->ddpo, p2 * KE);
+  put(a.weto, vg->weto, p2 * KE);
+  put(a.amsuo, vg->amsuo, u2 * KE);
+  put(a.dduo, vg->dduo, u2 * KE);
+  put(a.dtdxuo, vg->dtdxuo, u2);
+  put(a.dlyu, vg->dlyu, u2);
+  put(a.amsue, vg->amsue, v2 * KE);
+  put(a.ddue, vg->ddue, v2 * KE);
+  put(a.dpye, vg->dpye, v2);
+  put(a.dlxv, vg->dlxv, v2);
+  buf(h->d_uoo, u2 * KE);
+  buf(h->d_uoo_1, u2 * KE);
+  buf(h->d_voe, v2 * KE);
+  buf(h->d_voe_1, v2 * KE);
+  buf(h->d_wno, p2 * (KE + 1));
+  buf(h->d_wno_1, p2 * (KE + 1));
+  buf(h->d_zo, p2);
+  if (e == hipSuccess) e = ensure_fort(h);
+  if (e != hipSuccess) return map_err(note_error(e, "attach_vgrid", __LINE__));
+  a.uoo = h->d_uoo;
+  a.uoo_1 = h->d_uoo_1;
+  a.voe = h->d_voe;
+  a.voe_1 = h->d_voe_1;
+  a.wno = h->d_wno;
+  a.wno_1 = h->d_wno_1;
+  a.zo = h->d_zo;
+  h->ocv_ready = true;
+  return ORCTM_NH_OK;
+}
+
+int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, const double* voe,
+                        const double* voe_1, const double* wno, const double* wno_1, const double* zo,
+                        double* divg) {
+  if (!h || !uoo || !uoo_1 || !voe || !voe_1 || !wno || !wno_1 || !zo) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
+  if (!h->ocv_ready) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(h->opts.device);
+  const size_t IE = h->IE, JE = h->JE, KE = h->KE;
+  const size_t u3 = (IE - h->i_start + 1) * JE * KE, v3 = IE * (JE - h->j_start + 1) * KE, w3 = IE * JE * (KE + 1);
+  hipStream_t st = h->ctx.stream;
+  hipError_t e = hipMemcpyAsync(h->d_uoo, uoo, u3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_uoo_1, uoo_1, u3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_voe, voe, v3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_voe_1, voe_1, v3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_wno, wno, w3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_wno_1, wno_1, w3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_zo, zo, IE * JE * 8, hipMemcpyHostToDevice, st);
+  // DIVG into the Fortran staging array, then b = DIVG*WETO exactly as orctm_nh_set_rhs forms it
+  if (e == hipSuccess) e = launch_divergence(st, h->ocv, h->d_fort);
+  if (e == hipSuccess)
+    e = launch_rhs(st, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, h->d_fort, h->IE, h->JE);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess && divg) {
+    for (size_t k = 0; k < KE && e == hipSuccess; ++k) {
+      const size_t off = 1 + IE * (1 + JE * k);
+      e = hipMemcpy2D(divg + off, IE * 8, h->d_fort + off, IE * 8, (IE - 2) * 8, JE - 2, hipMemcpyDeviceToHost);
+    }
+  }
+  return map_err(e);
+}
+
+int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, double* wno) {
+  if (!h || !pnh || !uoo || !voe || !wno) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
+  if (!h->ocv_ready) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(h->opts.device);
+  const size_t IE = h->IE, JE = h->JE, KE = h->KE;
+  const size_t p3 = IE * JE * KE, u3 = (IE - h->i_start + 1) * JE * KE, v3 = IE * (JE - h->j_start + 1) * KE,
+               w3 = IE * JE * (KE + 1);
+  hipStream_t st = h->ctx.stream;
+  hipError_t e = hipMemcpyAsync(h->d_fort, pnh, p3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_uoo, uoo, u3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_voe, voe, v3 * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = launch_project(st, h->ocv, h->sys.p64, h->pitch, h->plane, h->d_fort, h->d_uoo, h->d_voe, h->d_wno);
+  if (e == hipSuccess) e = hipMemcpyAsync(pnh, h->d_fort, p3 * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(uoo, h->d_uoo, u3 * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(voe, h->d_voe, v3 * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(wno, h->d_wno, w3 * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   return map_err(e);
 }
 

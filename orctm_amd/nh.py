@@ -80,6 +80,48 @@ class NHSolver:
     def perturb_rhs(self, amplitude: float, seed: int) -> None:
         self._rc(self.lib.orctm_nh_perturb_rhs(self.h, amplitude, seed), "orctm_nh_perturb_rhs")
 
+    # -- the steps of OCVTOT_NH around the solve (ocvtot_nh.f90:11-62, 151-230) ------------------
+    def _vshape(self, kind: str):
+        IE, JE, KE = self.shape
+        NU, NV = IE - self.grid.i_start + 1, JE - self.grid.j_start + 1
+        return {"p2": (IE, JE), "p3": (IE, JE, KE), "u2": (NU, JE), "u3": (NU, JE, KE), "v2": (IE, NV),
+                "v3": (IE, NV, KE), "w3": (IE, JE, KE + 1)}[kind]
+
+    def _check_v(self, a: np.ndarray, kind: str, name: str) -> None:
+        shp = self._vshape(kind)
+        if a.dtype != np.float64 or a.shape != shp or not a.flags["F_CONTIGUOUS"]:
+            raise ValueError(f"{name} must be a Fortran-ordered float64 array of shape {shp}")
+
+    _VGRID = (("dlxp", "p2"), ("dlyp", "p2"), ("ddpo", "p3"), ("weto", "p3"), ("amsuo", "u3"), ("dduo", "u3"),
+              ("dtdxuo", "u2"), ("dlyu", "u2"), ("amsue", "v3"), ("ddue", "v3"), ("dpye", "v2"), ("dlxv", "v2"),
+              ("dtdxpo", "p2"), ("dtdyo", "p2"))
+
+    def attach_vgrid(self, v) -> None:
+        """Upload the velocity-side metrics (arrays named as in MO_COMMO1, e.g. from synth.make_velocity)."""
+        names = tuple(n for n, _ in self._VGRID)
+        for n, kind in self._VGRID:
+            self._check_v(getattr(v, n), kind, n)
+        vg = abi.VGridC(*[_dp(getattr(v, n)) for n in names], float(v.dt), float(v.dti))
+        self._vg_keep = v
+        self._rc(self.lib.orctm_nh_attach_vgrid(self.h, ctypes.byref(vg)), "orctm_nh_attach_vgrid")
+
+    def divergence(self, uoo, uoo_1, voe, voe_1, wno, wno_1, zo, divg: Optional[np.ndarray] = None) -> None:
+        """DIVG from the velocity tendencies (ocvtot_nh.f90:11-62); the resident b becomes DIVG*WETO."""
+        arrs = (uoo, uoo_1, voe, voe_1, wno, wno_1, zo)
+        for a, kind, n in zip(arrs, ("u3", "u3", "v3", "v3", "w3", "w3", "p2"),
+                              ("uoo", "uoo_1", "voe", "voe_1", "wno", "wno_1", "zo")):
+            self._check_v(a, kind, n)
+        if divg is not None:
+            self._check_f(divg, "divg")
+        self._rc(self.lib.orctm_nh_divergence(self.h, *[_dp(a) for a in arrs],
+                                              _dp(divg) if divg is not None else None), "orctm_nh_divergence")
+
+    def project(self, pnh, uoo, voe, wno) -> None:
+        """PNH halo, U/V pressure correction and WNO by continuity (ocvtot_nh.f90:151-230), in place."""
+        for a, kind, n in zip((pnh, uoo, voe, wno), ("p3", "u3", "v3", "w3"), ("pnh", "uoo", "voe", "wno")):
+            self._check_v(a, kind, n)
+        self._rc(self.lib.orctm_nh_project(self.h, _dp(pnh), _dp(uoo), _dp(voe), _dp(wno)), "orctm_nh_project")
+
     # -- the reference's three calls -------------------------------------------------------------
     def solve(self, divg: np.ndarray, pnh: np.ndarray, check: bool = True) -> dict:
         """petsc_solve_3D: b = DIVG*WETO, solve, write the PNH interior. Returns the statistics."""

@@ -282,7 +282,9 @@ def make_grid(spec: GridSpec) -> Grid:
     ddpo = np.where(lev == kb3 - 1, f[:, :, None] * dzw[None, None, :], ddpo)
     weto = np.asfortranarray(weto)
     ddpo = np.asfortranarray(ddpo)
-    return Grid(spec, dzw, dlxp, dlyp, dlxu, dlyv, ddpo, weto)
+    g = Grid(spec, dzw, dlxp, dlyp, dlxu, dlyv, ddpo, weto)
+    g.depto, g.tiestw, g.tiestu = depto, tiestw, tiestu
+    return g
 
 
 def make_divg(grid: Grid, seed: Optional[int] = None, sigma: Optional[float] = None) -> np.ndarray:
@@ -310,6 +312,96 @@ def make_divg(grid: Grid, seed: Optional[int] = None, sigma: Optional[float] = N
     divgv = (gv[:, :-1, :] - gv[:, 1:, :]) / grid.dlyp[:, :, None]          # J = 1..JE
     divgw = (gw[:, :, :KE] - gw[:, :, 1:]) / np.maximum(grid.ddpo, 1.0e-3)
     return np.asfortranarray(divgu + divgv + divgw)
+
+
+def _exch(a: np.ndarray, spec: GridSpec, i0: int = 1, j0: int = 1) -> None:
+    """bounds_exch (mo_parallel.f90:551-668) for one process, in place, on an array a(i0:IE, j0:JE[, k])
+    (numpy index = Fortran index - lower bound): with ICYCLI_X, A(1) = A(IE-1) and A(IE) = A(2); then
+    with ICYCLI_Y, A(j0) = A(JE-1) and A(JE) = A(j0+1) (j0 = 0 for v-points: A(0) and A(1) are used)."""
+    IE = spec.nx + 2
+    if spec.cyclic_x:
+        xs1, xs2 = a[2 - i0].copy(), a[IE - 1 - i0].copy()
+        a[1 - i0] = xs2
+        a[IE - i0] = xs1
+    if spec.cyclic_y:
+        y1, y2 = a[:, 1].copy(), a[:, -2].copy()
+        a[:, 0] = y2
+        a[:, -1] = y1
+
+
+class VelocityState:
+    """Velocity-side metrics and a seeded velocity state of OCVTOT_NH (ocvtot_nh.f90:11-62, 151-236),
+    all Fortran arrays with halos (mo_commo1.f90:105-146), I_start = J_start = 0:
+    u-points (0:IE, JE[, KE]), v-points (IE, 0:JE[, KE]), w-points (IE, JE, KEP)."""
+
+
+def make_velocity(grid: Grid, seed: Optional[int] = None, dt: float = 60.0, sigma: Optional[float] = None):
+    """Metrics by the model's own rules -- DEUTO = min(DEPTO(I), DEPTO(I+1)), DEUTE likewise in j, end
+    columns copied (boden.f90:116-133); DDUO, DDUE, AMSUO, AMSUE from them (boden.f90:195-211);
+    DTDXPO = DT/DLXP, DTDYO = DT/DLYP, DTDXUO = DT/DLXU, DPYE = DT/DLYV (beleg.f90:48-74); DTI = 1/DT --
+    and seeded velocities UOO, VOE, WNO (masked by AMSUO, AMSUE and wet tops) with their previous-step
+    values UOO_1, VOE_1, WNO_1 and a small sea-surface elevation ZO."""
+    spec = grid.spec
+    seed = spec.seed if seed is None else seed
+    sigma = spec.sigma_u if sigma is None else sigma
+    IE, JE, KE = grid.IE, grid.JE, grid.KE
+    dzw, tiestw, tiestu, depto = grid.dzw, grid.tiestw, grid.tiestu, grid.depto
+    v = VelocityState()
+    v.IE, v.JE, v.KE, v.i_start, v.j_start = IE, JE, KE, 0, 0
+    v.dt, v.dti = float(dt), 1.0 / float(dt)
+    deuto = np.zeros((IE + 1, JE), order="F")           # I = 0..IE
+    deuto[1:IE, :] = np.minimum(depto[:-1, :], depto[1:, :])
+    deuto[0, :] = deuto[1, :]
+    deuto[IE, :] = deuto[IE - 1, :]
+    _exch(deuto, spec, i0=0)
+    deute = np.zeros((IE, JE + 1), order="F")           # J = 0..JE
+    deute[:, 1:JE] = np.minimum(depto[:, :-1], depto[:, 1:])
+    deute[:, 0] = deute[:, 1]
+    deute[:, JE] = deute[:, JE - 1]
+    _exch(deute, spec, j0=0)
+
+    def layers(deut):
+        d = np.zeros(deut.shape + (KE,), order="F")
+        for k in range(KE):
+            full = tiestu[k + 1] < deut
+            part = (tiestu[k] < deut) & ~full
+            d[..., k] = np.where(full, dzw[k], np.where(part, deut - tiestw[k], 0.0))
+        return np.asfortranarray(d)
+
+    v.dduo, v.ddue = layers(deuto), layers(deute)
+    v.amsuo = np.asfortranarray((v.dduo > 0.0).astype(np.float64))
+    v.amsue = np.asfortranarray((v.ddue > 0.0).astype(np.float64))
+    v.dlxp, v.dlyp, v.ddpo, v.weto = grid.dlxp, grid.dlyp, grid.ddpo, grid.weto
+    v.dlyu = np.full((IE + 1, JE), spec.dy, order="F")
+    v.dlxv = np.empty((IE, JE + 1), order="F")
+    v.dlxv[:, 1:] = grid.dlxp
+    v.dlxv[:, 0] = grid.dlxp[:, 0]
+    v.dtdxpo = np.asfortranarray(dt / grid.dlxp)
+    v.dtdyo = np.asfortranarray(dt / grid.dlyp)
+    v.dtdxuo = np.asfortranarray(dt / grid.dlxu)
+    v.dpye = np.asfortranarray(dt / grid.dlyv)
+
+    def field(stream, shape, mask, scale):
+        n = int(np.prod(shape))
+        a = scale * normal(seed, stream, n).reshape(shape[::-1]).transpose(tuple(range(len(shape)))[::-1])
+        return np.asfortranarray(a * mask)
+
+    wtop = np.zeros((IE, JE, KE + 1), order="F")
+    wtop[:, :, :KE] = grid.weto
+    v.uoo = field(61, (IE + 1, JE, KE), v.amsuo, sigma)
+    v.uoo_1 = field(62, (IE + 1, JE, KE), v.amsuo, sigma)
+    v.voe = field(63, (IE, JE + 1, KE), v.amsue, sigma)
+    v.voe_1 = field(64, (IE, JE + 1, KE), v.amsue, sigma)
+    v.wno = field(65, (IE, JE, KE + 1), wtop, 0.01 * sigma)
+    v.wno_1 = field(66, (IE, JE, KE + 1), wtop, 0.01 * sigma)
+    v.zo = field(67, (IE, JE), grid.weto[:, :, 0], 0.5)
+    for a in (v.uoo, v.uoo_1):
+        _exch(a, spec, i0=0)
+    for a in (v.voe, v.voe_1):
+        _exch(a, spec, j0=0)
+    for a in (v.wno, v.wno_1, v.zo):
+        _exch(a, spec)
+    return v
 
 
 def build(spec: GridSpec, with_rhs: bool = True):
