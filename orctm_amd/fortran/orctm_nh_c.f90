@@ -6,6 +6,8 @@
 !   petsc_end       (mo_petsc.f90:131-143, called from orctm.f90:1585,1651)
 !
 ! In ocvtot_nh.f90 and orctm.f90 replace `use mo_petsc` by `use orctm_nh_c`; nothing else changes.
+! Optionally, ocvtot_nh_device replaces the body of OCVTOT_NH (ocvtot_nh.f90:11-249, single process,
+! CG2D4NONHY off): the divergence, the solve and the projection then all run on the device.
 ! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
 ! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
 ! One GPU holds the whole domain (nprocx = nprocy = 1). Build: gfortran/ifort -c orctm_nh_c.f90 and
@@ -14,8 +16,8 @@ MODULE orctm_nh_c
   USE, INTRINSIC :: ISO_C_BINDING
   IMPLICIT NONE
   PRIVATE
-  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end
-  PUBLIC :: orctm_nh_grid_t, orctm_nh_options_t, orctm_nh_stats_t
+  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end, ocvtot_nh_device
+  PUBLIC :: orctm_nh_grid_t, orctm_nh_options_t, orctm_nh_stats_t, orctm_nh_vgrid_t
 
   INTEGER(C_INT), PARAMETER, PUBLIC :: ORCTM_NH_OK = 0, ORCTM_NH_FP64 = 0, ORCTM_NH_MIXED = 1
 
@@ -38,7 +40,35 @@ MODULE orctm_nh_c
     REAL(C_DOUBLE) :: reserved(4)
   END TYPE
 
+  TYPE, BIND(C) :: orctm_nh_vgrid_t
+    TYPE(C_PTR) :: dlxp, dlyp, ddpo, weto, amsuo, dduo, dtdxuo, dlyu, amsue, ddue, dpye, dlxv
+    TYPE(C_PTR) :: dtdxpo, dtdyo
+    REAL(C_DOUBLE) :: dt, dti
+  END TYPE
+
   INTERFACE
+    INTEGER(C_INT) FUNCTION orctm_nh_attach_vgrid(handle, vg) BIND(C, name="orctm_nh_attach_vgrid")
+      IMPORT :: C_INT, C_PTR, orctm_nh_vgrid_t
+      TYPE(C_PTR), VALUE :: handle
+      TYPE(orctm_nh_vgrid_t), INTENT(IN) :: vg
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_divergence(handle, uoo, uoo_1, voe, voe_1, wno, wno_1, zo, divg) &
+        BIND(C, name="orctm_nh_divergence")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(IN) :: uoo(*), uoo_1(*), voe(*), voe_1(*), wno(*), wno_1(*), zo(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: divg(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_project(handle, pnh, uoo, voe, wno) BIND(C, name="orctm_nh_project")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(INOUT) :: pnh(*), uoo(*), voe(*), wno(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_solve_resident(handle, stats) BIND(C, name="orctm_nh_solve_resident")
+      IMPORT :: C_INT, C_PTR, orctm_nh_stats_t
+      TYPE(C_PTR), VALUE :: handle
+      TYPE(orctm_nh_stats_t), INTENT(OUT) :: stats
+    END FUNCTION
     SUBROUTINE orctm_nh_default_options(opts) BIND(C, name="orctm_nh_default_options")
       IMPORT :: orctm_nh_options_t
       TYPE(orctm_nh_options_t), INTENT(OUT) :: opts
@@ -63,6 +93,7 @@ MODULE orctm_nh_c
   END INTERFACE
 
   TYPE(C_PTR), SAVE :: nh_handle = C_NULL_PTR
+  LOGICAL, SAVE :: nh_vgrid_attached = .FALSE.
 
 CONTAINS
 
@@ -99,11 +130,44 @@ CONTAINS
     IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_solve status ', ierr, ' rel.res ', st%rel_res
   END SUBROUTINE petsc_solve_3D
 
+  ! The whole of OCVTOT_NH (ocvtot_nh.f90:1-259, single process, CG2D4NONHY off) on the device:
+  ! DIVG from the velocity tendencies (:11-62), the solve (:150), the PNH halo, the U/V correction
+  ! and WNO by continuity (:151-230); the copies of :232-249 stay here on the host.
+  SUBROUTINE ocvtot_nh_device
+    USE MO_PARAM1, ONLY: IE, JE, KE, KEP, I_start, J_start
+    USE MO_COMMO1, ONLY: DLXP, DLYP, DDPO, WETO, AMSUO, DDUO, DTDXUO, DLYU, AMSUE, DDUE, DPYE, DLXV, &
+                         DTDXPO, DTDYO, DT, DTI, UOO, UOO_1, VOE, VOE_1, WNO, WNO_1, WO, ZO, DIVG, PNH
+    TYPE(orctm_nh_vgrid_t) :: vg
+    TYPE(orctm_nh_stats_t) :: st
+    INTEGER(C_INT) :: ierr
+    IF (.NOT. nh_vgrid_attached) THEN
+      vg%dlxp = C_LOC(DLXP); vg%dlyp = C_LOC(DLYP); vg%ddpo = C_LOC(DDPO); vg%weto = C_LOC(WETO)
+      vg%amsuo = C_LOC(AMSUO); vg%dduo = C_LOC(DDUO); vg%dtdxuo = C_LOC(DTDXUO); vg%dlyu = C_LOC(DLYU)
+      vg%amsue = C_LOC(AMSUE); vg%ddue = C_LOC(DDUE); vg%dpye = C_LOC(DPYE); vg%dlxv = C_LOC(DLXV)
+      vg%dtdxpo = C_LOC(DTDXPO); vg%dtdyo = C_LOC(DTDYO)
+      vg%dt = DT; vg%dti = DTI
+      ierr = orctm_nh_attach_vgrid(nh_handle, vg)
+      IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_attach_vgrid status ', ierr
+      nh_vgrid_attached = .TRUE.
+    ENDIF
+    ierr = orctm_nh_divergence(nh_handle, UOO, UOO_1, VOE, VOE_1, WNO, WNO_1, ZO, DIVG)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_divergence status ', ierr
+    ierr = orctm_nh_solve_resident(nh_handle, st)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_solve_resident status ', ierr, ' rel.res ', st%rel_res
+    ierr = orctm_nh_project(nh_handle, PNH, UOO, VOE, WNO)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_project status ', ierr
+    UOO_1(:,:,:) = UOO(:,:,:)
+    VOE_1(:,:,:) = VOE(:,:,:)
+    WNO_1(:,:,:) = WNO(:,:,:)
+    WO(:,:,:) = WNO(:,:,:)
+  END SUBROUTINE ocvtot_nh_device
+
   ! mo_petsc.f90:131-143
   SUBROUTINE petsc_end
     INTEGER(C_INT) :: ierr
     IF (C_ASSOCIATED(nh_handle)) ierr = orctm_nh_destroy(nh_handle)
     nh_handle = C_NULL_PTR
+    nh_vgrid_attached = .FALSE.
   END SUBROUTINE petsc_end
 
 END MODULE orctm_nh_c
