@@ -75,12 +75,13 @@ def pct(xs, q):
     return xs[k]
 
 
-def time_solves(solver, steps, warmup):
+def time_solves(solver, steps, warmup, profiled=True):
     for _ in range(warmup):
         st = solver.solve_resident()
         if st["rc"] != abi.OK:
             raise RuntimeError(f"warmup solve failed: {st} {abi.last_error()}")
-    solver.profile(reset=True)
+    if profiled:
+        solver.profile(reset=True)
     lat, stats = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -91,7 +92,7 @@ def time_solves(solver, steps, warmup):
         if st["rc"] != abi.OK:
             raise RuntimeError(f"solve failed: {st} {abi.last_error()}")
     wall = time.perf_counter() - t0
-    return wall, lat, stats, solver.profile(reset=True)
+    return wall, lat, stats, (solver.profile(reset=True) if profiled else None)
 
 
 def cpu_baseline(g, divg, seconds_hint=30.0):
@@ -143,16 +144,20 @@ def main():
     results = {}
     sols = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        s = NHSolver(g, options(precision=prec, warm_start=False, profile=True, tile_rows=args.tile_rows))
+        # timed solves: the production path (inner iterations replayed from the captured hipGraph)
+        s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows))
         s.set_rhs(divg)
         log(f"[bench] {name}: device bytes {s.device_bytes() / 1e9:.2f} GB")
-        wall, lat, stats, prof = time_solves(s, args.steps, args.warmup)
+        wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
         s.get_solution(pnh)
         sols[name] = g.interior(pnh)
+        # kernel timing: the same solves with HIP events around every inner kernel (direct launches)
+        s.set_profile(True)
+        _, _, pstats, prof = time_solves(s, 2, 1)
+        s.close()
         results[name] = dict(wall=wall, lat=lat, stats=stats, prof=prof)
         log(f"[bench] {name}: {args.steps / wall:.3f} solves/s, last {stats[-1]}, prof {prof}")
-        s.close()
 
     mx, f64 = results["mixed"], results["fp64"]
     K = args.steps
@@ -248,7 +253,7 @@ def bench_batch(args):
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         solvers = []
         for g, divg in systems:
-            s = NHSolver(g, options(precision=prec, warm_start=False, profile=True, tile_rows=args.tile_rows))
+            s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows))
             s.set_rhs(divg)
             solvers.append(s)
         batch = NHBatch(solvers)
@@ -256,7 +261,6 @@ def bench_batch(args):
             rc, st = batch.solve()
             if rc != abi.OK:
                 raise RuntimeError(f"batch warmup failed: {rc} {abi.last_error()}")
-        batch.profile(reset=True)
         lat = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -266,7 +270,17 @@ def bench_batch(args):
             if rc != abi.OK:
                 raise RuntimeError(f"batch solve failed: {rc}")
         wall = time.perf_counter() - t0
+        batch.close()
+        # kernel timing: the same batch with HIP events around every inner kernel (direct launches)
+        for s in solvers:
+            s.set_profile(True)
+        batch = NHBatch(solvers)
+        batch.solve()
+        batch.profile(reset=True)
+        rc, pst = batch.solve()
         prof = batch.profile(reset=True)
+        for s in solvers:
+            s.set_profile(False)
         inner = sum(x["inner_iters"] for x in st)
         # cell-iterations actually processed per solve (converged systems drop out of later launches)
         cell_iters = sum(x["inner_iters"] * g.cells for x, (g, _) in zip(st, systems))
@@ -292,8 +306,8 @@ def bench_batch(args):
     mx, f64 = out["mixed"], out["fp64"]
     n = len(systems)
     prof = mx["prof"]
-    # bytes moved by all matvec launches of the timed solves / their summed device time
-    mv_gbs = mx["mv_bytes"] * args.steps / (prof["matvec_ms"] * 1e-3) / 1e9
+    # bytes moved by the matvec launches of one profiled batch solve / their summed device time
+    mv_gbs = mx["mv_bytes"] / (prof["matvec_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
@@ -336,9 +350,9 @@ def bench_transect(args):
     cpu_s = (time.perf_counter() - t) / reps
     res = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        s = NHSolver(g, options(precision=prec, warm_start=False, profile=True))
+        s = NHSolver(g, options(precision=prec, warm_start=False))
         s.set_rhs(divg)
-        wall, lat, stats, prof = time_solves(s, args.steps, args.warmup)
+        wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
         s.get_solution(pnh)
         res[name] = dict(wall=wall, lat=lat, stats=stats,

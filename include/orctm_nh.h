@@ -186,6 +186,10 @@ typedef struct orctm_nh_profile {
   double reserved[2];
 } orctm_nh_profile_t;
 int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* prof, int32_t reset);
+/* Turns kernel timing on or off between solves (rebuilds the handle's solver context; a batch
+ * created afterwards profiles if any member does). With timing on, the inner kernels are launched
+ * directly with HIP events around them instead of from the captured graph. */
+int orctm_nh_set_profile(orctm_nh_t* h, int32_t on);
 
 /* Streaming bandwidth of the device: a plain 16-B-per-lane copy of `bytes` repeated `reps` times,
  * timed with HIP events; *gbps = bytes read + written per second / 1e9. The copy is also the

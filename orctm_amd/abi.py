@@ -99,6 +99,7 @@ EXPORTS = [
     "orctm_nh_state_bytes", "orctm_nh_server_default_options", "orctm_nh_server_create", "orctm_nh_server_submit",
     "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
     "orctm_nh_server_destroy", "orctm_nh_attach_vgrid", "orctm_nh_divergence", "orctm_nh_project",
+    "orctm_nh_set_profile",
 ]
 
 _lib = None

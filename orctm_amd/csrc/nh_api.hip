@@ -293,82 +293,101 @@ void order_into(std::vector<Tile>& nat, const Sys& s, int q, int h) {
   for (const Tile& t : system_tiles(q, s.nx, s.ny, h)) nat.push_back(t);
 }
 
-// Times rounds of (matvec, column update) over the given systems at each candidate height and
-// returns the fastest. The kernels run a full iteration's memory traffic against scratch inner
-// states (not the first iteration, alpha = beta = 0): they overwrite the inner work vectors w, p,
-// q and the partial sums, all of which every solve re-initialises before use, and store x and r
-// back unchanged; the FP64 solution, the right-hand side and the operator are not touched.
-hipError_t tune_height(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& members, int* best) {
-  const int m = (int)members.size();
-  std::vector<Sys> ts(m);
-  std::vector<char> hin(inner_size(c.fp32) * m, 0);
-  for (int q = 0; q < m; ++q) {
-    if (c.fp32) {
-      Inner<float>* in = reinterpret_cast<Inner<float>*>(hin.data()) + q;
-      in->done = RUN;
-      in->rho = 1.f;
-    } else {
-      Inner<double>* in = reinterpret_cast<Inner<double>*>(hin.data()) + q;
-      in->done = RUN;
-      in->rho = 1.0;
-    }
-  }
+// Scratch copies of some systems of a context, for timing the inner kernels under a tile-height
+// assignment. The kernels run a full iteration's memory traffic against scratch inner states (not
+// the first iteration, alpha = beta = 0): they overwrite the inner work vectors w, p, q and the
+// partial sums, all of which every solve re-initialises before use, and store x and r back
+// unchanged; the FP64 solution, the right-hand side and the operator are not touched.
+struct TuneSet {
+  Ctx& c;
+  std::vector<Sys> ts;
   Sys* d_ts = nullptr;
   void* d_in = nullptr;
-  Tile* d_t = nullptr;
-  hipError_t e = hipMalloc(&d_ts, sizeof(Sys) * m);
-  if (e == hipSuccess) e = hipMalloc(&d_in, hin.size());
-  if (e == hipSuccess) e = hipMemcpy(d_in, hin.data(), hin.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    for (int q = 0; q < m; ++q) {
-      ts[q] = dev[members[q]];
-      ts[q].inner = static_cast<char*>(d_in) + inner_size(c.fp32) * q;
-    }
-    e = hipMemcpy(d_ts, ts.data(), sizeof(Sys) * m, hipMemcpyHostToDevice);
+  TuneSet(Ctx& ctx, const std::vector<Sys>& dev, const std::vector<int>& members) : c(ctx) {
+    ts.resize(members.size());
+    for (size_t q = 0; q < members.size(); ++q) ts[q] = dev[members[q]];
   }
-  float best_ms = -1.f;
-  *best = 12;
-  for (int hc : kTileHeights) {
-    if (e != hipSuccess) break;
-    std::vector<Tile> nat;
-    for (int q = 0; q < m; ++q) order_into(nat, ts[q], q, hc);
-    std::vector<Tile> ord = xcd_order(nat);
-    const int nt = (int)ord.size();
-    if (d_t) hipFree(d_t);
-    d_t = nullptr;
-    e = hipMalloc(&d_t, sizeof(Tile) * std::max(nt, 1));
-    if (e == hipSuccess && nt) e = hipMemcpy(d_t, ord.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice);
-    // one warm round, then 3 timed rounds
-    float ms = 0.f;
+  ~TuneSet() {
+    if (d_ts) hipFree(d_ts);
+    if (d_in) hipFree(d_in);
+  }
+  hipError_t upload() {
+    const size_t m = ts.size(), isz = inner_size(c.fp32);
+    std::vector<char> hin(isz * m, 0);
+    for (size_t q = 0; q < m; ++q) {
+      if (c.fp32) {
+        Inner<float>* in = reinterpret_cast<Inner<float>*>(hin.data()) + q;
+        in->done = RUN;
+        in->rho = 1.f;
+      } else {
+        Inner<double>* in = reinterpret_cast<Inner<double>*>(hin.data()) + q;
+        in->done = RUN;
+        in->rho = 1.0;
+      }
+    }
+    hipError_t e = hipMalloc(&d_ts, sizeof(Sys) * m);
+    if (e == hipSuccess) e = hipMalloc(&d_in, hin.size());
+    if (e == hipSuccess) e = hipMemcpy(d_in, hin.data(), hin.size(), hipMemcpyHostToDevice);
+    for (size_t q = 0; q < m; ++q) ts[q].inner = static_cast<char*>(d_in) + isz * q;
+    if (e == hipSuccess) e = hipMemcpy(d_ts, ts.data(), sizeof(Sys) * m, hipMemcpyHostToDevice);
+    return e;
+  }
+  // one warm round, then 3 timed rounds of (matvec, column update), heights h[q] per system
+  hipError_t time(const std::vector<int>& h, float* ms) {
+    std::vector<Tile*> d_tiles;
+    std::vector<int> nts, hs;
+    hipError_t e = hipSuccess;
+    for (int hc : kTileHeights) {
+      std::vector<Tile> nat;
+      for (size_t q = 0; q < ts.size(); ++q)
+        if (h[q] == hc) order_into(nat, ts[q], (int)q, hc);
+      if (nat.empty()) continue;
+      std::vector<Tile> ord = xcd_order(nat);
+      Tile* d = nullptr;
+      e = hipMalloc(&d, sizeof(Tile) * ord.size());
+      if (e != hipSuccess) break;
+      d_tiles.push_back(d);
+      nts.push_back((int)ord.size());
+      hs.push_back(hc);
+      e = hipMemcpy(d, ord.data(), sizeof(Tile) * ord.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) break;
+    }
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
       if (rep == 1) e = hipEventRecord(c.ev0, c.stream);
-      if (e == hipSuccess) e = launch_bwd(c.stream, c.fp32, hc, d_ts, d_t, nt, 0);
-      if (e == hipSuccess) e = launch_fwd(c.stream, c.fp32, false, hc, d_ts, d_t, nt);
+      for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
+        e = launch_bwd(c.stream, c.fp32, hs[g], d_ts, d_tiles[g], nts[g], 0);
+      for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
+        e = launch_fwd(c.stream, c.fp32, false, hs[g], d_ts, d_tiles[g], nts[g]);
     }
     if (e == hipSuccess) e = hipEventRecord(c.ev1, c.stream);
     if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c.ev0, c.ev1);
-    if (e == hipSuccess && (best_ms < 0.f || ms < best_ms)) {
-      best_ms = ms;
-      *best = hc;
-    }
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, c.ev0, c.ev1);
+    for (Tile* d : d_tiles) hipFree(d);
+    return e;
   }
-  if (d_t) hipFree(d_t);
-  if (d_ts) hipFree(d_ts);
-  if (d_in) hipFree(d_in);
-  return note_error(e, "tile-height tuning", __LINE__);
-}
+};
 
-// Chooses each system's tile height (forced[q] if nonzero, else tuned per size class) and builds
-// the iteration tile groups, one per height in use, each in XCD order.
+// Chooses each system's tile height and builds the iteration tile groups, one per height in use,
+// each in XCD order. forced[q] > 0 fixes a system's height. The others are tuned in two stages:
+// (1) per size class, timing the class's systems alone at each height; (2) for a context of several
+// systems, the whole launch is timed under the per-class choice and under each uniform height (all
+// systems run together, so a class that prefers small tiles alone may not inside a large batch),
+// and the fastest assignment is kept. Both choices are cached for the life of the process, keyed
+// by device, precision, size class (ceil(log2) of the column count; of the total for stage 2) and
+// ceil(log2) of the number of systems.
 hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& forced) {
   c.heights.assign(c.nsys, 12);
   std::map<int, std::vector<int>> classes;
+  std::vector<int> tuned;
+  long long cols = 0;
   for (int q = 0; q < c.nsys; ++q) {
-    if (forced[q] > 0)
+    if (forced[q] > 0) {
       c.heights[q] = forced[q];
-    else
+    } else {
       classes[size_class(dev[q])].push_back(q);
+      tuned.push_back(q);
+      cols += (long long)dev[q].nx * dev[q].ny;
+    }
   }
   for (auto& kv : classes) {
     const auto key = std::make_tuple(c.device, c.fp32 ? 1 : 0, kv.first, ceil_log2((long long)kv.second.size()));
@@ -379,11 +398,52 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       if (it != g_tune.end()) h = it->second;
     }
     if (!h) {
-      HIPCHK(tune_height(c, dev, kv.second, &h));
+      TuneSet ts(c, dev, kv.second);
+      HIPCHK(ts.upload());
+      float best = -1.f;
+      for (int hc : kTileHeights) {
+        float ms = 0.f;
+        HIPCHK(ts.time(std::vector<int>(kv.second.size(), hc), &ms));
+        if (best < 0.f || ms < best) {
+          best = ms;
+          h = hc;
+        }
+      }
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune[key] = h;
     }
     for (int q : kv.second) c.heights[q] = h;
+  }
+  if (tuned.size() > 1 && classes.size() > 0) {
+    // stage 2: the whole launch; code 0 = keep the per-class choice, else a uniform height
+    const auto key = std::make_tuple(c.device, 100 + (c.fp32 ? 1 : 0), ceil_log2(cols), ceil_log2((long long)tuned.size()));
+    int code = -1;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(key);
+      if (it != g_tune.end()) code = it->second;
+    }
+    if (code < 0) {
+      TuneSet ts(c, dev, tuned);
+      HIPCHK(ts.upload());
+      std::vector<int> mix(tuned.size());
+      for (size_t q = 0; q < tuned.size(); ++q) mix[q] = c.heights[tuned[q]];
+      float best = 0.f;
+      HIPCHK(ts.time(mix, &best));
+      code = 0;
+      for (int hc : kTileHeights) {
+        float ms = 0.f;
+        HIPCHK(ts.time(std::vector<int>(tuned.size(), hc), &ms));
+        if (ms < best) {
+          best = ms;
+          code = hc;
+        }
+      }
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = code;
+    }
+    if (code > 0)
+      for (int q : tuned) c.heights[q] = code;
   }
   for (int hc : kTileHeights) {
     std::vector<Tile> nat;
@@ -1315,6 +1375,21 @@ int orctm_nh_profile_read(orctm_nh_t* h, orctm_nh_profile_t* p, int32_t reset) {
   if (!h) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
   return profile_read(h->ctx, p, reset);
+}
+
+int orctm_nh_set_profile(orctm_nh_t* h, int32_t on) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
+  const int want = on ? 1 : 0;
+  if ((h->opts.profile != 0) == (want != 0)) return ORCTM_NH_OK;
+  hipSetDevice(h->opts.device);
+  if (h->ctx.stream) hipStreamSynchronize(h->ctx.stream);
+  ctx_destroy(h->ctx);
+  h->opts.profile = want;
+  hipError_t e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, h->fp32, h->opts.check_every, h->opts.max_outer,
+                           h->opts.device, want != 0);
+  return map_err(e);
 }
 
 int orctm_nh_batch_profile_read(orctm_nh_batch_t* b, orctm_nh_profile_t* p, int32_t reset) {

@@ -77,6 +77,10 @@ class NHSolver:
         self._rc(self.lib.orctm_nh_clone(self.h, ctypes.byref(h)), "orctm_nh_clone")
         return NHSolver(self.grid, self.opts, _handle=h)
 
+    def set_profile(self, on: bool) -> None:
+        """Kernel timing on/off between solves (orctm_nh_set_profile)."""
+        self._rc(self.lib.orctm_nh_set_profile(self.h, 1 if on else 0), "orctm_nh_set_profile")
+
     def perturb_rhs(self, amplitude: float, seed: int) -> None:
         self._rc(self.lib.orctm_nh_perturb_rhs(self.h, amplitude, seed), "orctm_nh_perturb_rhs")
 
