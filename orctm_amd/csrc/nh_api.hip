@@ -52,12 +52,13 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::vector<Group> groups;       // inner-iteration tilings, one per tile height in use
   std::vector<int> heights;        // tile height of each system
-  hipGraphExec_t graph = nullptr;
+  std::map<int, hipGraphExec_t> graphs;  // captured runs of 1, 2, 4, ... inner iterations
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional kernel timing: events around every matvec / update launch (direct launches)
   bool prof = false;
   std::vector<hipEvent_t> kev;      // 4 per iteration of a chunk
   std::vector<int> iter_seen;       // per system, inner iterations already accounted
+  std::vector<double> lrate;        // per system, mean log-reduction of r.r per inner iteration (0: unknown)
   double mv_ms = 0.0, up_ms = 0.0;
   long long mv_n = 0, up_n = 0, cells = 0;
 };
@@ -239,7 +240,7 @@ std::vector<Tile> xcd_order(const std::vector<Tile>& nat) {
 }
 
 void ctx_destroy(Ctx& c) {
-  if (c.graph) hipGraphExecDestroy(c.graph);
+  for (auto& kv : c.graphs) hipGraphExecDestroy(kv.second);
   if (c.d_sys) hipFree(c.d_sys);
   if (c.d_tiles) hipFree(c.d_tiles);
   for (Group& g : c.groups)
@@ -468,6 +469,7 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   c.nsys = (int)systems.size();
   c.prof = prof;
   c.iter_seen.assign(c.nsys, 0);
+  c.lrate.assign(c.nsys, 0.0);
   for (const Sys& s : systems) c.cells += (long long)s.nx * s.ny * s.nz;
   c.fp32 = fp32;
   c.check_every = check_every > 0 ? check_every : 16;
@@ -505,36 +507,86 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   return ctx_heights(c, dev, forced);
 }
 
-// check_every iterations (matvec + column update) captured once, replayed per chunk. With kernel
-// timing on, the same launches are issued directly with HIP events around each phase's kernels.
-hipError_t ctx_chunk(Ctx& c) {
+// n inner iterations (matvec + column update each). Runs of a power-of-two length are captured
+// once into a hipGraph and replayed; n is issued as its binary decomposition, largest run first.
+// With kernel timing on, the same launches are issued directly with HIP events around each phase.
+hipError_t ctx_chunk(Ctx& c, int n) {
   if (c.prof) {
-    for (int it = 0; it < c.check_every; ++it) {
+    for (int it = 0; it < n; ++it) {
       HIPCHK(iter_bwd(c, 0, c.kev[4 * it], c.kev[4 * it + 1]));
       HIPCHK(iter_fwd(c, false, c.kev[4 * it + 2], c.kev[4 * it + 3]));
     }
     return hipSuccess;
   }
-  if (!c.graph) {
-    DEVLOCK;
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (int it = 0; it < c.check_every && e == hipSuccess; ++it) {
-      e = iter_bwd(c, 0);
-      if (e == hipSuccess) e = iter_fwd(c, false);
+  for (int len = 1 << 30; len > 0; len >>= 1) {
+    if (!(n & len)) continue;
+    hipGraphExec_t& ge = c.graphs[len];
+    if (!ge) {
+      DEVLOCK;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      for (int it = 0; it < len && e == hipSuccess; ++it) {
+        e = iter_bwd(c, 0);
+        if (e == hipSuccess) e = iter_fwd(c, false);
+      }
+      hipError_t e2 = hipStreamEndCapture(c.stream, &g);
+      if (e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        c.graphs.erase(len);
+        return e;
+      }
+      if (e2 != hipSuccess) c.graphs.erase(len);
+      HIPCHK(e2);
+      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) c.graphs.erase(len);
+      HIPCHK(e);
     }
-    hipError_t e2 = hipStreamEndCapture(c.stream, &g);
-    if (e != hipSuccess) {
-      if (g) hipGraphDestroy(g);
-      return e;
-    }
-    HIPCHK(e2);
-    e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    HIPCHK(e);
+    HIPCHK(hipGraphLaunch(ge, c.stream));
   }
-  return hipGraphLaunch(c.graph, c.stream);
+  return hipSuccess;
+}
+
+// Inner iterations to issue before the next poll: enough for every running system to reach its
+// tolerance if its squared residual keeps falling at its mean rate (log(rr/rr0)/iter, from this
+// inner solve once it has run, else from the system's previous inner solve), at least 1 and at most
+// check_every. Launches past a system's convergence exit at entry, but each still costs a few
+// microseconds, and a run of them drains the queue faster than the runtime submits the graph's
+// nodes (a stall of about a millisecond), so the chunk is sized to the work left. The first chunk
+// of an inner solve uses the tolerance the reduction kernel sets, max(rtol_inner, rtol/2 |b|/|r|)^2,
+// computed here from the outer state read at the start of the refinement step.
+int chunk_length(Ctx& c, bool first) {
+  int need = 1;
+  for (int q = 0; q < c.nsys; ++q) {
+    int n = c.check_every;
+    if (first) {
+      const Outer& o = c.h_outer[q];
+      if (o.odone != RUN) continue;
+      double tol = 0.5 * o.rtol * o.nb / o.nr;
+      if (tol < o.rtol_inner) tol = o.rtol_inner;
+      if (c.lrate[q] < 0.0 && tol > 0.0 && tol < 1.0) n = (int)std::ceil(2.0 * std::log(tol) / c.lrate[q]);
+    } else {
+      if (inner_done(c, q) != RUN) continue;
+      double rr, rr0, tol2;
+      if (c.fp32) {
+        const Inner<float>& in = reinterpret_cast<const Inner<float>*>(c.h_inner)[q];
+        rr = in.rr, rr0 = in.rr0, tol2 = in.tol2;
+      } else {
+        const Inner<double>& in = reinterpret_cast<const Inner<double>*>(c.h_inner)[q];
+        rr = in.rr, rr0 = in.rr0, tol2 = in.tol2;
+      }
+      const int it = inner_iter(c, q);
+      const double red = rr / rr0;
+      if (it > 0 && red > 0.0 && red < 1.0 && tol2 > 0.0 && std::isfinite(red)) {
+        c.lrate[q] = std::log(red) / it;
+        const double left = std::log(tol2 / red) / c.lrate[q];
+        n = left <= 1.0 ? 1 : (int)std::ceil(left);
+      }
+    }
+    need = std::max(need, std::min(std::max(n, 1), c.check_every));
+  }
+  return need;
 }
 
 // Runs the whole refinement for every system of the context; results stay on the device.
@@ -550,11 +602,14 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
     if (!any) break;
     HIPCHK(iter_fwd(c, true));
     std::fill(c.iter_seen.begin(), c.iter_seen.end(), 0);
-    // every inner solve ends within maxit iterations (device-side NOTCONV); the host bound below
-    // only guarantees termination of this loop whatever the device flags say
-    const int max_chunks = c.maxit / c.check_every + 2;
-    for (int chunk = 0; chunk < max_chunks; ++chunk) {
-      HIPCHK(ctx_chunk(c));
+    // every inner solve ends within maxit iterations (device-side NOTCONV); the host bound on the
+    // iterations issued only guarantees termination of this loop whatever the device flags say
+    const long long max_issue = (long long)c.maxit + 2LL * c.check_every;
+    long long issued = 0;
+    for (int chunk = 0; issued < max_issue; ++chunk) {
+      const int n = chunk_length(c, chunk == 0);
+      issued += n;
+      HIPCHK(ctx_chunk(c, n));
       HIPCHK(hipMemcpyAsync(c.h_inner, c.d_inner, inner_size(c.fp32) * c.nsys, hipMemcpyDeviceToHost, c.stream));
       HIPCHK(hipStreamSynchronize(c.stream));
       bool run = false;
@@ -564,9 +619,19 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
         const int it = inner_iter(c, q);
         worked = std::max(worked, it - c.iter_seen[q]);
         c.iter_seen[q] = it;
+        double rr, rr0;
+        if (c.fp32) {
+          const Inner<float>& in = reinterpret_cast<const Inner<float>*>(c.h_inner)[q];
+          rr = in.rr, rr0 = in.rr0;
+        } else {
+          const Inner<double>& in = reinterpret_cast<const Inner<double>*>(c.h_inner)[q];
+          rr = in.rr, rr0 = in.rr0;
+        }
+        const double red = rr / rr0;
+        if (it > 0 && red > 0.0 && red < 1.0 && std::isfinite(red)) c.lrate[q] = std::log(red) / it;
       }
       if (c.prof) {
-        for (int it = 0; it < std::min(worked, c.check_every); ++it) {
+        for (int it = 0; it < std::min(worked, n); ++it) {
           float a = 0.f, b = 0.f;
           HIPCHK(hipEventElapsedTime(&a, c.kev[4 * it], c.kev[4 * it + 1]));
           HIPCHK(hipEventElapsedTime(&b, c.kev[4 * it + 2], c.kev[4 * it + 3]));
