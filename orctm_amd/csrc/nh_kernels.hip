@@ -57,16 +57,30 @@ __device__ __forceinline__ float half_val(uint16_t b) { return (float)__builtin_
 
 constexpr float kHalfMin = 6.103515625e-05f;  // 2^-14
 
+// Streaming accesses of the inner kernels: every value is touched once per launch and re-read only
+// by the next launch, a gigabyte or more later, so they carry the non-temporal hint (no cache
+// residency is expected; the stores do not evict lines other blocks still read).
+template <typename T>
+__device__ __forceinline__ T ld_nt(gptr<const T> p, size_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(gptr<T> p, size_t i, T v) {
+  __builtin_nontemporal_store(v, p + i);
+}
+
 // Factor storage per inner precision: fp16 bits for FP32, the value itself for FP64.
 template <typename T>
 struct Fac {
   using S = T;
   __device__ static T ld(gptr<const S> p, size_t i) { return p[i]; }
+  __device__ static T ld_nt(gptr<const S> p, size_t i) { return onh::ld_nt(p, i); }
 };
 template <>
 struct Fac<float> {
   using S = uint16_t;
   __device__ static float ld(gptr<const uint16_t> p, size_t i) { return half_val(p[i]); }
+  __device__ static float ld_nt(gptr<const uint16_t> p, size_t i) { return half_val(onh::ld_nt(p, i)); }
 };
 
 // Sum of a and b over a block of NW wavefronts; valid in thread 0.
@@ -560,10 +574,10 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         rin[idx] = T(0);
         continue;
       }
-      const double r = G<const double>(s.b)[idx] - apply_a64(s, cc, G<const double>(s.p64), i, j, k, idx);
+      const double r = ld_nt<double>(G<const double>(s.b), idx) - apply_a64(s, cc, G<const double>(s.p64), i, j, k, idx);
       acc = fma(r, r, acc);
       const double S = cc[6] / G<const double>(s.zp)[idx];
-      rin[idx] = (T)((S * (-r)) * gamma);
+      st_nt<T>(rin, idx, (T)((S * (-r)) * gamma));
     }
   }
   publish_rows<double, TY, TY>(s, t.bx, t.by, acc, dummy);
@@ -590,7 +604,8 @@ __global__ __launch_bounds__(NT) void k_refine(const Sys* __restrict__ systems, 
 #pragma unroll 4
   for (int k = 0; k < s.nz; ++k) {
     const size_t idx = col + (size_t)k * s.plane;
-    const T xk = pend ? fmad(alpha, pl[idx], x[idx]) : x[idx];
+    const T xv = ld_nt<T>(x, idx);
+    const T xk = pend ? fmad(alpha, ld_nt<T>(pl, idx), xv) : xv;
     p64[idx] = p64[idx] + (double)xk * g;
   }
 }
@@ -648,10 +663,10 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
       for (int u = 0; u < U; ++u) {
         const int k = k0 + u < nz ? k0 + u : nz - 1;
         const size_t idx = col + (size_t)k * P;
-        b.r[u] = r[idx];
-        b.l[u] = Fac<T>::ld(L, idx);
-        b.d[u] = Fac<T>::ld(invD, idx) * cs;
-        if (!INIT) b.q[u] = q[idx];
+        b.r[u] = ld_nt<T>(r, idx);
+        b.l[u] = Fac<T>::ld_nt(L, idx);
+        b.d[u] = Fac<T>::ld_nt(invD, idx) * cs;
+        if (!INIT) b.q[u] = ld_nt<T>(q, idx);
       }
     };
     T yprev = T(0);
@@ -662,16 +677,16 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
           const size_t idx = col + (size_t)(k0 + u) * P;
           T rk = b.r[u];
           if (INIT) {
-            x[idx] = T(0);
+            st_nt<T>(x, idx, T(0));
           } else {
             rk = fmad(-alpha, b.q[u], rk);
-            r[idx] = rk;
+            st_nt<T>(r, idx, rk);
           }
           rr = fmad(rk, rk, rr);
           const T y = fmad(-b.l[u], yprev, rk);
           const T wv = y * b.d[u];
           rho = fmad(y, wv, rho);
-          w[idx] = wv;
+          st_nt<T>(w, idx, wv);
           yprev = y;
         }
       }
@@ -784,14 +799,14 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     st.w = w[om];
     st.l = Fac<T>::ld(L, om);
     st.p = pold[om];
-    if (centre && xupd) st.x = xv[om];
+    if (centre && xupd) st.x = ld_nt<T>(xv, om);
   };
   auto load_c = [&](int m, Stage& st) {
     if (!centre) return;
     const size_t cm = (size_t)(m > 0 ? m : 0) * P + off;
-    st.zw = zwi[cm];
-    st.zp = zpi[cm];
-    st.f = flg[cm];
+    st.zw = ld_nt<T>(zwi, cm);
+    st.zp = ld_nt<T>(zpi, cm);
+    st.f = ld_nt<uint16_t>(flg, cm);
   };
   // back substitution and search-direction update of level m for this thread's column
   auto level = [&](int m, const Stage& st) -> T {
@@ -802,8 +817,8 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     const T pout = val ? pm : T(0);
     if (slot) lds[m % 3][sy][sx] = pout;
     if (active) {
-      pnew[(size_t)m * P + off] = pm;
-      if (xupd) xv[(size_t)m * P + off] = fmad(alpha, st.p, st.x);
+      st_nt<T>(pnew, (size_t)m * P + off, pm);
+      if (xupd) st_nt<T>(xv, (size_t)m * P + off, fmad(alpha, st.p, st.x));
     }
     return pout;
   };
@@ -826,7 +841,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
       acc = fmad(c7[5], pm1, acc);
       acc = fmad(c7[6], pp1, acc);
       if (active) {
-        q[(size_t)k * P + off] = acc;
+        st_nt<T>(q, (size_t)k * P + off, acc);
         dot = fmad(p0, acc, dot);
       }
     }
