@@ -369,13 +369,24 @@ struct TuneSet {
 };
 
 // Chooses each system's tile height and builds the iteration tile groups, one per height in use,
-// each in XCD order. forced[q] > 0 fixes a system's height. The others are tuned in two stages:
+// each in XCD order. forced[q] > 0 fixes a system's height. Systems (or contexts) large enough to
+// fill the device with 64 x 12 tiles take 12 rows untimed. The others are tuned in two stages:
 // (1) per size class, timing the class's systems alone at each height; (2) for a context of several
 // systems, the whole launch is timed under the per-class choice and under each uniform height (all
 // systems run together, so a class that prefers small tiles alone may not inside a large batch),
 // and the fastest assignment is kept. Both choices are cached for the life of the process, keyed
 // by device, precision, size class (ceil(log2) of the column count; of the total for stage 2) and
 // ceil(log2) of the number of systems.
+// Tiles of 64 x 12 needed before a set of systems fills the device by itself (8 blocks per CU):
+// beyond it the tallest tile (fewest halo re-reads) is the fastest and no timing is run.
+constexpr long long kFillTiles = 8 * 256;
+
+long long tiles12(const std::vector<Sys>& dev, const std::vector<int>& members) {
+  long long n = 0;
+  for (int q : members) n += (long long)((dev[q].nx + TX - 1) / TX) * ((dev[q].ny + 11) / 12);
+  return n;
+}
+
 hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<int>& forced) {
   c.heights.assign(c.nsys, 12);
   std::map<int, std::vector<int>> classes;
@@ -398,6 +409,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       auto it = g_tune.find(key);
       if (it != g_tune.end()) h = it->second;
     }
+    if (!h && tiles12(dev, kv.second) >= kFillTiles) h = 12;
     if (!h) {
       TuneSet ts(c, dev, kv.second);
       HIPCHK(ts.upload());
@@ -424,6 +436,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       auto it = g_tune.find(key);
       if (it != g_tune.end()) code = it->second;
     }
+    if (code < 0 && tiles12(dev, tuned) >= kFillTiles) code = 12;
     if (code < 0) {
       TuneSet ts(c, dev, tuned);
       HIPCHK(ts.upload());
