@@ -555,29 +555,85 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   if (o->odone != RUN) return;
-  Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
   const double nb = o->nb;
   const bool zero = (nb == 0.0);
   const double gamma = 1.0 / (o->smax * o->nr_prev);
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
   double acc = 0.0, dummy = 0.0;
   if (i < s.nx && j < s.ny) {
-    const size_t col = (size_t)j * s.pitch + i;
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    const size_t col = (size_t)j * s.pitch + i, P = s.plane, IP = s.pitch;
     gptr<T> rin = G<T>(s.r);
-    double cc[7];
-    load_col(s, col, cc);
-#pragma unroll 2
-    for (int k = 0; k < s.nz; ++k) {
-      const size_t idx = col + (size_t)k * s.plane;
-      if (zero) {
-        s.p64[idx] = 0.0;
-        rin[idx] = T(0);
-        continue;
+    gptr<double> p64w = G<double>(s.p64);
+    if (zero) {
+      for (int k = 0; k < nz; ++k) {
+        p64w[col + (size_t)k * P] = 0.0;
+        rin[col + (size_t)k * P] = T(0);
       }
-      const double r = ld_nt<double>(G<const double>(s.b), idx) - apply_a64(s, cc, G<const double>(s.p64), i, j, k, idx);
-      acc = fma(r, r, acc);
-      const double S = cc[6] / G<const double>(s.zp)[idx];
-      st_nt<T>(rin, idx, (T)((S * (-r)) * gamma));
+    } else {
+      gptr<const double> p = G<const double>(s.p64);
+      gptr<const double> b = G<const double>(s.b);
+      gptr<const double> zw = G<const double>(s.zw);
+      gptr<const double> zp = G<const double>(s.zp);
+      gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+      double cc[7];
+      load_col(s, col, cc);
+      // horizontal neighbour columns (periodic wrap where cyclic; absent ones read this column and
+      // are replaced by +0), the same for every level
+      const bool hw = i > 0 || s.cyc_x, he = i < nx - 1 || s.cyc_x;
+      const bool hn = j > 0 || s.cyc_y, hs = j < ny - 1 || s.cyc_y;
+      const size_t cw = i > 0 ? col - 1 : (s.cyc_x ? col + (size_t)(nx - 1) : col);
+      const size_t ce = i < nx - 1 ? col + 1 : (s.cyc_x ? col - (size_t)(nx - 1) : col);
+      const size_t cn = j > 0 ? col - IP : (s.cyc_y ? col + IP * (size_t)(ny - 1) : col);
+      const size_t cs = j < ny - 1 ? col + IP : (s.cyc_y ? col - IP * (size_t)(ny - 1) : col);
+      // level data in two register sets used alternately: level k+1 is loaded while level k is
+      // computed; p and ZW of the column itself rotate through registers (x(k-1), x(k), ZW(k))
+      struct Lv {
+        double b, zp, zw1, pd, pw, pe, pn, ps;
+        unsigned f;
+      };
+      auto load = [&](int k, Lv& v) {
+        const int kk = k < nz ? k : nz - 1;
+        const size_t o = (size_t)kk * P;
+        v.b = ld_nt<double>(b, col + o);
+        v.zp = ld_nt<double>(zp, col + o);
+        v.zw1 = ld_nt<double>(zw, col + o + P);
+        v.f = ld_nt<uint16_t>(flg, col + o);
+        v.pd = p[col + (kk < nz - 1 ? o + P : o)];
+        v.pw = p[cw + o];
+        v.pe = p[ce + o];
+        v.pn = p[cn + o];
+        v.ps = p[cs + o];
+      };
+      double xu = 0.0, x0 = p[col], zwk = ld_nt<double>(zw, col);
+      auto compute = [&](int k, const Lv& v) {
+        const double xd = k < nz - 1 ? v.pd : 0.0;
+        double c7[7];
+        row_coef<double>(cc, zwk, v.zw1, v.zp, v.f, c7);
+        double a = c7[0] * x0;
+        a = fma(c7[1], hw ? v.pw : 0.0, a);
+        a = fma(c7[2], he ? v.pe : 0.0, a);
+        a = fma(c7[3], hn ? v.pn : 0.0, a);
+        a = fma(c7[4], hs ? v.ps : 0.0, a);
+        a = fma(c7[5], xu, a);
+        a = fma(c7[6], xd, a);
+        const double r = v.b - a;
+        acc = fma(r, r, acc);
+        const double S = cc[6] / v.zp;
+        st_nt<T>(rin, col + (size_t)k * P, (T)((S * (-r)) * gamma));
+        xu = x0;
+        x0 = xd;
+        zwk = v.zw1;
+      };
+      Lv l0, l1;
+      load(0, l0);
+      for (int k = 0; k < nz; k += 2) {
+        load(k + 1, l1);
+        compute(k, l0);
+        if (k + 1 >= nz) break;
+        load(k + 2, l0);
+        compute(k + 1, l1);
+      }
     }
   }
   publish_rows<double, TY, TY>(s, t.bx, t.by, acc, dummy);
