@@ -240,6 +240,43 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
 int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, double* wno);
 
 /* ------------------------------------------------------------------------------------------
+ * The barotropic free-surface solve of the hydrostatic core: ITPREP (ORCTM/src_orctm/itprep.f90:15-38,
+ * called once from orctm.f90:872) and TRONEU (troneu.f90:48-131, called every step from orctm.f90:1414),
+ * for one process holding the whole domain. Arrays are the caller's Fortran arrays with halos
+ * (mo_commo1.f90:105-146): p-points (IE,JE), u-points (I_start:IE,JE), v-points (IE,J_start:JE). */
+typedef struct orctm_nh_sor_grid {
+  int32_t ie, je, i_start, j_start;
+  int32_t cyclic_x, cyclic_y;  /* ICYCLI_X, ICYCLI_Y */
+  int32_t iterations;          /* red-black sweeps per solve (250 in troneu.f90:103) */
+  int32_t device;              /* HIP device ordinal */
+  const double* dlxp;          /* DLXP(IE,JE) */
+  const double* dlyp;          /* DLYP(IE,JE) */
+  const double* dtdxpo;        /* DTDXPO(IE,JE) = DT/DLXP */
+  const double* dtdyo;         /* DTDYO(IE,JE) = DT/DLYP */
+  const double* weto;          /* WETO(IE,JE,KE): the first layer is used */
+  const double* dlxu;          /* DLXU(I_start:IE,JE) */
+  const double* dlyu;          /* DLYU(I_start:IE,JE) */
+  const double* deuto;         /* DEUTO(I_start:IE,JE), u-point depth (boden.f90:116-122) */
+  const double* dlxv;          /* DLXV(IE,J_start:JE) */
+  const double* dlyv;          /* DLYV(IE,J_start:JE) */
+  const double* deute;         /* DEUTE(IE,J_start:JE), v-point depth (boden.f90:124-130) */
+  double g, conn, cono, stabn; /* G, CONN, CONO = 1-CONN, STABN (orctm.f90:657-661) */
+  double dt;                   /* DT */
+  double sorpar;               /* SORPAR, the over-relaxation factor (trotest.f90), in (0, 2) */
+} orctm_nh_sor_grid_t;
+typedef struct orctm_nh_sor orctm_nh_sor_t;
+/* Uploads the metrics and forms XX, UF, VF, FF on the device (itprep.f90:15-38). */
+int orctm_nh_sor_create(const orctm_nh_sor_grid_t* grid, orctm_nh_sor_t** sor);
+/* troneu.f90:48-131: B1O from U1O, UZO (I_start:IE,JE) and V1E, VZE (IE,J_start:JE) on J = J_start+1..JE,
+ * I = I_start+1..IE (b1o is read, for the entries the source does not recompute, and written); Z1O
+ * starts at 0 with its boundary rows and columns at B1O/XX, then `iterations` red-black SOR sweeps,
+ * each half-sweep followed by the one-process bounds_exch (cyclic copies). *device_ms (may be NULL)
+ * receives the device time from B1O to the last sweep. */
+int orctm_nh_troneu(orctm_nh_sor_t* sor, const double* u1o, const double* uzo, const double* v1e,
+                    const double* vze, double* b1o, double* z1o, double* device_ms);
+int orctm_nh_sor_destroy(orctm_nh_sor_t* sor);
+
+/* ------------------------------------------------------------------------------------------
  * Building blocks of the inner solve, exposed for verification. No counterpart in the reference
  * (its Krylov internals live inside PETSc). "inner" is the scaled operator C = S (-A),
  * S = DLXP*DLYP/ZP per cell, held in FP32 (MIXED) or FP64 (FP64); x, y, r, z are interior vectors

@@ -297,3 +297,59 @@ def project(grid, v, pnh, uoo=None, voe=None):
     wn = np.zeros((v.IE, v.JE, v.KE + 1), order="F")
     L.orc_project(ctypes.byref(g), _d(p), _d(u), _d(w_), _d(wn))
     return p, u, w_, wn
+
+
+# ---- barotropic SOR: ITPREP / TRONEU (nh_oracle_sor.c) -------------------------------------------
+
+
+class _SorGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("IE", "JE", "i_start", "j_start", "cyc_x", "cyc_y", "iterations")] + \
+               [(n, _dp) for n in ("dlxp", "dlyp", "dtdxpo", "dtdyo", "weto1", "dlxu", "dlyu", "deuto", "dlxv",
+                                   "dlyv", "deute")] + \
+               [(n, ctypes.c_double) for n in ("g", "conn", "cono", "stabn", "dt", "sorpar")]
+
+
+def _sorgrid(b):
+    g = _SorGrid(b.IE, b.JE, b.i_start, b.j_start, int(b.cyc_x), int(b.cyc_y), int(b.iterations))
+    for n in ("dlxp", "dlyp", "dtdxpo", "dtdyo", "weto1", "dlxu", "dlyu", "deuto", "dlxv", "dlyv", "deute"):
+        a = getattr(b, n)
+        assert a.flags.f_contiguous and a.dtype == np.float64, n
+        setattr(g, n, a.ctypes.data_as(_dp))
+    for n in ("g", "conn", "cono", "stabn", "dt", "sorpar"):
+        setattr(g, n, float(getattr(b, n)))
+    g._keep = b
+    return g
+
+
+def _sor_lib():
+    L = lib()
+    if not getattr(L, "_sor", False):
+        L.orc_itprep.argtypes = [ctypes.POINTER(_SorGrid)] + [_dp] * 4
+        L.orc_troneu.argtypes = [ctypes.POINTER(_SorGrid)] + [_dp] * 10
+        L._sor = True
+    return L
+
+
+def itprep(b):
+    """XX (IE,JE), UF (I_start:IE,JE), VF (IE,J_start:JE), FF (IE,JE) of itprep.f90:15-38."""
+    L = _sor_lib()
+    g = _sorgrid(b)
+    IE, JE = b.IE, b.JE
+    xx = np.zeros((IE, JE), order="F")
+    uf = np.zeros((IE - b.i_start + 1, JE), order="F")
+    vf = np.zeros((IE, JE - b.j_start + 1), order="F")
+    ff = np.zeros((IE, JE), order="F")
+    L.orc_itprep(ctypes.byref(g), _d(xx), _d(uf), _d(vf), _d(ff))
+    return xx, uf, vf, ff
+
+
+def troneu(b, coef, b1o=None):
+    """Z1O (IE,JE) and B1O after troneu.f90:48-131 with the velocities held by `b`."""
+    L = _sor_lib()
+    g = _sorgrid(b)
+    xx, uf, vf, ff = coef
+    b1 = np.zeros((b.IE, b.JE), order="F") if b1o is None else np.array(b1o, order="F", copy=True)
+    z = np.zeros((b.IE, b.JE), order="F")
+    L.orc_troneu(ctypes.byref(g), _d(xx), _d(uf), _d(vf), _d(ff), _d(b.u1o), _d(b.uzo), _d(b.v1e), _d(b.vze),
+                 _d(b1), _d(z))
+    return z, b1

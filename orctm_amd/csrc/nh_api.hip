@@ -22,9 +22,15 @@
 
 #include "../../include/orctm_nh.h"
 #include "nh_kernels.h"
+#include "nh_lock.h"
 #include "nh_ocv.h"
 
 using namespace onh;
+
+std::recursive_mutex& onh::device_lock() {
+  static std::recursive_mutex mu;
+  return mu;
+}
 
 namespace {
 
@@ -128,8 +134,7 @@ thread_local char g_last_error[256] = "";
 // thread-local capture mode. Every capture holds this lock, and so does every entry point that
 // allocates, frees, copies on the null stream or synchronises the device, so that handles can be
 // created and used from several threads (the serving loop runs its own thread).
-std::recursive_mutex g_dev_mu;
-#define DEVLOCK std::lock_guard<std::recursive_mutex> dev_lock_(g_dev_mu)
+#define DEVLOCK std::lock_guard<std::recursive_mutex> dev_lock_(onh::device_lock())
 
 hipError_t note_error(hipError_t e, const char* what, int line) {
   if (e != hipSuccess)

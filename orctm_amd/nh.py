@@ -361,3 +361,58 @@ def petsc_end() -> None:
     s = _state.pop("solver", None)
     if s is not None:
         s.close()
+
+
+class NHBarotropic:
+    """The barotropic free-surface solve (ITPREP at construction, TRONEU per call) on one GPU.
+    `b` holds the Fortran arrays with halos and the constants (e.g. from synth.make_barotropic)."""
+
+    _PTRS = ("dlxp", "dlyp", "dtdxpo", "dtdyo", "weto", "dlxu", "dlyu", "deuto", "dlxv", "dlyv", "deute")
+
+    def __init__(self, b, weto=None, device: int = 0):
+        self.lib = abi.load()
+        self.b = b
+        arrs = {n: getattr(b, n) for n in self._PTRS if n != "weto"}
+        arrs["weto"] = b.weto1 if weto is None else weto
+        for n, a in arrs.items():
+            if a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+                raise ValueError(f"{n} must be a Fortran-ordered float64 array")
+        self._keep = arrs
+        g = abi.SorGridC(b.IE, b.JE, b.i_start, b.j_start, int(b.cyc_x), int(b.cyc_y), int(b.iterations), device,
+                         *[_dp(arrs[n]) for n in self._PTRS], float(b.g), float(b.conn), float(b.cono),
+                         float(b.stabn), float(b.dt), float(b.sorpar))
+        h = ctypes.c_void_p()
+        rc = self.lib.orctm_nh_sor_create(ctypes.byref(g), ctypes.byref(h))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_sor_create")
+        self.h = h
+
+    def troneu(self, u1o, uzo, v1e, vze, b1o, z1o) -> float:
+        """Z1O (and B1O) of troneu.f90:48-131, in place; returns the device time in ms."""
+        for a in (u1o, uzo, v1e, vze, b1o, z1o):
+            if a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+                raise ValueError("arrays must be Fortran-ordered float64")
+        ms = ctypes.c_double()
+        rc = self.lib.orctm_nh_troneu(self.h, _dp(u1o), _dp(uzo), _dp(v1e), _dp(vze), _dp(b1o), _dp(z1o),
+                                      ctypes.byref(ms))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_troneu")
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.orctm_nh_sor_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -368,6 +368,7 @@ def make_velocity(grid: Grid, seed: Optional[int] = None, dt: float = 60.0, sigm
             d[..., k] = np.where(full, dzw[k], np.where(part, deut - tiestw[k], 0.0))
         return np.asfortranarray(d)
 
+    v.deuto, v.deute = deuto, deute
     v.dduo, v.ddue = layers(deuto), layers(deute)
     v.amsuo = np.asfortranarray((v.dduo > 0.0).astype(np.float64))
     v.amsue = np.asfortranarray((v.ddue > 0.0).astype(np.float64))
@@ -402,6 +403,49 @@ def make_velocity(grid: Grid, seed: Optional[int] = None, dt: float = 60.0, sigm
     for a in (v.wno, v.wno_1, v.zo):
         _exch(a, spec)
     return v
+
+
+class BarotropicState:
+    """Inputs of the barotropic free-surface solve (itprep.f90, troneu.f90) as Fortran arrays with
+    halos, I_start = J_start = 0: metrics, constants and a seeded velocity state."""
+
+
+def make_barotropic(grid: Grid, seed: Optional[int] = None, dt: float = 60.0, sorpar: float = 1.75,
+                    iterations: int = 250):
+    """Metrics by the model's rules (u/v-point depths DEUTO, DEUTE as in boden.f90:116-133; DTDXPO,
+    DTDYO from beleg.f90:48-49; G = 9.81, CONN = 0.5, CONO = 1 - CONN, STABN = 0.6 as orctm.f90:657-661)
+    and seeded barotropic velocities U1O, UZO (masked by AMSUO(:,:,1)) and V1E, VZE (by AMSUE(:,:,1))."""
+    spec = grid.spec
+    seed = spec.seed if seed is None else seed
+    v = make_velocity(grid, seed=seed, dt=dt)
+    IE, JE = grid.IE, grid.JE
+    b = BarotropicState()
+    b.IE, b.JE, b.i_start, b.j_start = IE, JE, 0, 0
+    b.cyc_x, b.cyc_y, b.iterations = bool(spec.cyclic_x), bool(spec.cyclic_y), int(iterations)
+    b.g, b.conn, b.stabn, b.dt, b.sorpar = 9.81, 0.5, 0.6, float(dt), float(sorpar)
+    b.cono = 1.0 - b.conn
+    b.dlxp, b.dlyp, b.dtdxpo, b.dtdyo = grid.dlxp, grid.dlyp, v.dtdxpo, v.dtdyo
+    b.weto1 = np.asfortranarray(grid.weto[:, :, 0])
+    b.dlxu, b.dlyu, b.deuto = grid.dlxu, v.dlyu, v.deuto
+    b.dlyv, b.dlxv, b.deute = grid.dlyv, v.dlxv, v.deute
+
+    def field(stream, shape, mask, scale):
+        n = int(np.prod(shape))
+        a = scale * normal(seed, stream, n).reshape(shape[::-1]).T
+        return np.asfortranarray(a * mask)
+
+    # transports exist where the face has a wet top layer (AMSUO(:,:,1), AMSUE(:,:,1), boden.f90:195-211)
+    mu = v.amsuo[:, :, 0]
+    mv = v.amsue[:, :, 0]
+    b.u1o = field(81, (IE + 1, JE), mu, spec.sigma_u)
+    b.uzo = field(82, (IE + 1, JE), mu, spec.sigma_u)
+    b.v1e = field(83, (IE, JE + 1), mv, spec.sigma_u)
+    b.vze = field(84, (IE, JE + 1), mv, spec.sigma_u)
+    for a in (b.u1o, b.uzo):
+        _exch(a, spec, i0=0)
+    for a in (b.v1e, b.vze):
+        _exch(a, spec, j0=0)
+    return b
 
 
 def build(spec: GridSpec, with_rhs: bool = True):
