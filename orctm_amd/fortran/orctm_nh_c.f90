@@ -7,7 +7,8 @@
 !
 ! In ocvtot_nh.f90 and orctm.f90 replace `use mo_petsc` by `use orctm_nh_c`; nothing else changes.
 ! Optionally, ocvtot_nh_device replaces the body of OCVTOT_NH (ocvtot_nh.f90:11-249, single process,
-! CG2D4NONHY off): the divergence, the solve and the projection then all run on the device.
+! CG2D4NONHY off): the divergence, the solve and the projection then all run on the device; and
+! troneu_device replaces CALL TRONEU (orctm.f90:1414): the barotropic SOR solve on the device.
 ! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
 ! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
 ! One GPU holds the whole domain (nprocx = nprocy = 1). Build: gfortran/ifort -c orctm_nh_c.f90 and
@@ -16,7 +17,7 @@ MODULE orctm_nh_c
   USE, INTRINSIC :: ISO_C_BINDING
   IMPLICIT NONE
   PRIVATE
-  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end, ocvtot_nh_device
+  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end, ocvtot_nh_device, troneu_device
   PUBLIC :: orctm_nh_grid_t, orctm_nh_options_t, orctm_nh_stats_t, orctm_nh_vgrid_t
 
   INTEGER(C_INT), PARAMETER, PUBLIC :: ORCTM_NH_OK = 0, ORCTM_NH_FP64 = 0, ORCTM_NH_MIXED = 1
@@ -40,6 +41,12 @@ MODULE orctm_nh_c
     REAL(C_DOUBLE) :: reserved(4)
   END TYPE
 
+  TYPE, BIND(C) :: orctm_nh_sor_grid_t
+    INTEGER(C_INT32_T) :: ie, je, i_start, j_start, cyclic_x, cyclic_y, iterations, device
+    TYPE(C_PTR) :: dlxp, dlyp, dtdxpo, dtdyo, weto, dlxu, dlyu, deuto, dlxv, dlyv, deute
+    REAL(C_DOUBLE) :: g, conn, cono, stabn, dt, sorpar
+  END TYPE
+
   TYPE, BIND(C) :: orctm_nh_vgrid_t
     TYPE(C_PTR) :: dlxp, dlyp, ddpo, weto, amsuo, dduo, dtdxuo, dlyu, amsue, ddue, dpye, dlxv
     TYPE(C_PTR) :: dtdxpo, dtdyo
@@ -47,6 +54,19 @@ MODULE orctm_nh_c
   END TYPE
 
   INTERFACE
+    INTEGER(C_INT) FUNCTION orctm_nh_sor_create(sg, sor) BIND(C, name="orctm_nh_sor_create")
+      IMPORT :: C_INT, C_PTR, orctm_nh_sor_grid_t
+      TYPE(orctm_nh_sor_grid_t), INTENT(IN) :: sg
+      TYPE(C_PTR), INTENT(OUT) :: sor
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_troneu(sor, u1o, uzo, v1e, vze, b1o, z1o, device_ms) &
+        BIND(C, name="orctm_nh_troneu")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: sor
+      REAL(C_DOUBLE), INTENT(IN) :: u1o(*), uzo(*), v1e(*), vze(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: b1o(*), z1o(*)
+      TYPE(C_PTR), VALUE :: device_ms
+    END FUNCTION
     INTEGER(C_INT) FUNCTION orctm_nh_attach_vgrid(handle, vg) BIND(C, name="orctm_nh_attach_vgrid")
       IMPORT :: C_INT, C_PTR, orctm_nh_vgrid_t
       TYPE(C_PTR), VALUE :: handle
@@ -93,6 +113,7 @@ MODULE orctm_nh_c
   END INTERFACE
 
   TYPE(C_PTR), SAVE :: nh_handle = C_NULL_PTR
+  TYPE(C_PTR), SAVE :: sor_handle = C_NULL_PTR
   LOGICAL, SAVE :: nh_vgrid_attached = .FALSE.
 
 CONTAINS
@@ -161,6 +182,32 @@ CONTAINS
     WNO_1(:,:,:) = WNO(:,:,:)
     WO(:,:,:) = WNO(:,:,:)
   END SUBROUTINE ocvtot_nh_device
+
+  ! TRONEU (troneu.f90:48-131, single process) on the device; the first call also does the work of
+  ! ITPREP (itprep.f90:15-38) there. Replace `CALL TRONEU` (orctm.f90:1414) by `CALL troneu_device`.
+  SUBROUTINE troneu_device
+    USE MO_PARAM1, ONLY: IE, JE, I_start, J_start, ICYCLI_X, ICYCLI_Y
+    USE MO_PARA2, ONLY: SORPAR
+    USE MO_COMMO1, ONLY: DLXP, DLYP, DTDXPO, DTDYO, WETO, DLXU, DLYU, DEUTO, DLXV, DLYV, DEUTE, G, CONN, CONO, &
+                         STABN, DT, U1O, UZO, V1E, VZE, B1O, Z1O
+    TYPE(orctm_nh_sor_grid_t) :: sg
+    INTEGER(C_INT) :: ierr
+    IF (.NOT. C_ASSOCIATED(sor_handle)) THEN
+      sg%ie = IE; sg%je = JE; sg%i_start = I_start; sg%j_start = J_start
+      sg%cyclic_x = ICYCLI_X; sg%cyclic_y = ICYCLI_Y; sg%iterations = 250; sg%device = 0
+      sg%dlxp = C_LOC(DLXP); sg%dlyp = C_LOC(DLYP); sg%dtdxpo = C_LOC(DTDXPO); sg%dtdyo = C_LOC(DTDYO)
+      sg%weto = C_LOC(WETO); sg%dlxu = C_LOC(DLXU); sg%dlyu = C_LOC(DLYU); sg%deuto = C_LOC(DEUTO)
+      sg%dlxv = C_LOC(DLXV); sg%dlyv = C_LOC(DLYV); sg%deute = C_LOC(DEUTE)
+      sg%g = G; sg%conn = CONN; sg%cono = CONO; sg%stabn = STABN; sg%dt = DT; sg%sorpar = SORPAR
+      ierr = orctm_nh_sor_create(sg, sor_handle)
+      IF (ierr /= ORCTM_NH_OK) THEN
+        PRINT *, 'orctm_nh_sor_create failed, status ', ierr
+        STOP
+      ENDIF
+    ENDIF
+    ierr = orctm_nh_troneu(sor_handle, U1O, UZO, V1E, VZE, B1O, Z1O, C_NULL_PTR)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_troneu status ', ierr
+  END SUBROUTINE troneu_device
 
   ! mo_petsc.f90:131-143
   SUBROUTINE petsc_end
