@@ -249,6 +249,9 @@ typedef struct orctm_nh_sor_grid {
   int32_t cyclic_x, cyclic_y;  /* ICYCLI_X, ICYCLI_Y */
   int32_t iterations;          /* red-black sweeps per solve (250 in troneu.f90:103) */
   int32_t device;              /* HIP device ordinal */
+  int32_t half_sweeps;         /* half-sweeps per blocked device launch: 0 = the fastest of 4, 6, 8, 12, timed at
+                                  creation; or one of them. Results are bit-identical for every choice */
+  int32_t reserved;            /* 0 */
   const double* dlxp;          /* DLXP(IE,JE) */
   const double* dlyp;          /* DLYP(IE,JE) */
   const double* dtdxpo;        /* DTDXPO(IE,JE) = DT/DLXP */
@@ -274,6 +277,8 @@ int orctm_nh_sor_create(const orctm_nh_sor_grid_t* grid, orctm_nh_sor_t** sor);
  * receives the device time from B1O to the last sweep. */
 int orctm_nh_troneu(orctm_nh_sor_t* sor, const double* u1o, const double* uzo, const double* v1e,
                     const double* vze, double* b1o, double* z1o, double* device_ms);
+/* The half-sweeps per blocked launch in use and the kernel launches of one call's sweeps. */
+int orctm_nh_sor_info(orctm_nh_sor_t* sor, int32_t* half_sweeps, int32_t* launches);
 int orctm_nh_sor_destroy(orctm_nh_sor_t* sor);
 
 /* ------------------------------------------------------------------------------------------

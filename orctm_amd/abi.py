@@ -91,7 +91,7 @@ class VGridC(ctypes.Structure):
 class SorGridC(ctypes.Structure):
     """orctm_nh_sor_grid_t: metrics and constants of the barotropic SOR solve (ITPREP / TRONEU)."""
     _fields_ = [(n, ctypes.c_int32) for n in ("ie", "je", "i_start", "j_start", "cyclic_x", "cyclic_y",
-                                             "iterations", "device")] + \
+                                             "iterations", "device", "half_sweeps", "reserved")] + \
                [(n, ctypes.POINTER(ctypes.c_double)) for n in ("dlxp", "dlyp", "dtdxpo", "dtdyo", "weto", "dlxu",
                                                               "dlyu", "deuto", "dlxv", "dlyv", "deute")] + \
                [(n, ctypes.c_double) for n in ("g", "conn", "cono", "stabn", "dt", "sorpar")]
@@ -108,7 +108,8 @@ EXPORTS = [
     "orctm_nh_state_bytes", "orctm_nh_server_default_options", "orctm_nh_server_create", "orctm_nh_server_submit",
     "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
     "orctm_nh_server_destroy", "orctm_nh_attach_vgrid", "orctm_nh_divergence", "orctm_nh_project",
-    "orctm_nh_set_profile", "orctm_nh_sor_create", "orctm_nh_troneu", "orctm_nh_sor_destroy",
+    "orctm_nh_set_profile", "orctm_nh_sor_create", "orctm_nh_troneu", "orctm_nh_sor_info",
+    "orctm_nh_sor_destroy",
 ]
 
 _lib = None

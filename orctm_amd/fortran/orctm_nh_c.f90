@@ -43,6 +43,7 @@ MODULE orctm_nh_c
 
   TYPE, BIND(C) :: orctm_nh_sor_grid_t
     INTEGER(C_INT32_T) :: ie, je, i_start, j_start, cyclic_x, cyclic_y, iterations, device
+    INTEGER(C_INT32_T) :: half_sweeps, reserved
     TYPE(C_PTR) :: dlxp, dlyp, dtdxpo, dtdyo, weto, dlxu, dlyu, deuto, dlxv, dlyv, deute
     REAL(C_DOUBLE) :: g, conn, cono, stabn, dt, sorpar
   END TYPE
@@ -195,6 +196,7 @@ CONTAINS
     IF (.NOT. C_ASSOCIATED(sor_handle)) THEN
       sg%ie = IE; sg%je = JE; sg%i_start = I_start; sg%j_start = J_start
       sg%cyclic_x = ICYCLI_X; sg%cyclic_y = ICYCLI_Y; sg%iterations = 250; sg%device = 0
+      sg%half_sweeps = 0; sg%reserved = 0
       sg%dlxp = C_LOC(DLXP); sg%dlyp = C_LOC(DLYP); sg%dtdxpo = C_LOC(DTDXPO); sg%dtdyo = C_LOC(DTDYO)
       sg%weto = C_LOC(WETO); sg%dlxu = C_LOC(DLXU); sg%dlyu = C_LOC(DLYU); sg%deuto = C_LOC(DEUTO)
       sg%dlxv = C_LOC(DLXV); sg%dlyv = C_LOC(DLYV); sg%deute = C_LOC(DEUTE)

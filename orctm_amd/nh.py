@@ -369,7 +369,7 @@ class NHBarotropic:
 
     _PTRS = ("dlxp", "dlyp", "dtdxpo", "dtdyo", "weto", "dlxu", "dlyu", "deuto", "dlxv", "dlyv", "deute")
 
-    def __init__(self, b, weto=None, device: int = 0):
+    def __init__(self, b, weto=None, device: int = 0, half_sweeps: int = 0):
         self.lib = abi.load()
         self.b = b
         arrs = {n: getattr(b, n) for n in self._PTRS if n != "weto"}
@@ -379,7 +379,7 @@ class NHBarotropic:
                 raise ValueError(f"{n} must be a Fortran-ordered float64 array")
         self._keep = arrs
         g = abi.SorGridC(b.IE, b.JE, b.i_start, b.j_start, int(b.cyc_x), int(b.cyc_y), int(b.iterations), device,
-                         *[_dp(arrs[n]) for n in self._PTRS], float(b.g), float(b.conn), float(b.cono),
+                         int(half_sweeps), 0, *[_dp(arrs[n]) for n in self._PTRS], float(b.g), float(b.conn), float(b.cono),
                          float(b.stabn), float(b.dt), float(b.sorpar))
         h = ctypes.c_void_p()
         rc = self.lib.orctm_nh_sor_create(ctypes.byref(g), ctypes.byref(h))
@@ -398,6 +398,14 @@ class NHBarotropic:
         if rc != abi.OK:
             raise NHError(rc, "orctm_nh_troneu")
         return ms.value
+
+    def info(self) -> dict:
+        """Half-sweeps per blocked launch in use, and kernel launches per call's sweeps."""
+        hs, nl = ctypes.c_int32(), ctypes.c_int32()
+        rc = self.lib.orctm_nh_sor_info(self.h, ctypes.byref(hs), ctypes.byref(nl))
+        if rc != abi.OK:
+            raise NHError(rc, "orctm_nh_sor_info")
+        return {"half_sweeps": hs.value, "launches": nl.value}
 
     def close(self) -> None:
         if getattr(self, "h", None):

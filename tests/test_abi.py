@@ -62,7 +62,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
               "orctm_nh_server_options_t": (abi.ServerOptionsC, ["hbm_budget", "max_active", "device"]),
               "orctm_nh_request_stats_t": (abi.RequestStatsC, ["status", "rel_res", "t_done_ms"]),
               "orctm_nh_vgrid_t": (abi.VGridC, ["dlxp", "amsuo", "dtdyo", "dt", "dti"]),
-              "orctm_nh_sor_grid_t": (abi.SorGridC, ["iterations", "device", "dlxp", "deute", "g", "sorpar"])}
+              "orctm_nh_sor_grid_t": (abi.SorGridC, ["iterations", "device", "half_sweeps", "dlxp", "deute", "g",
+                                                     "sorpar"])}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/orctm_nh.h"', "int main(void){"]
     for cname, (_, fields) in checks.items():
         lines.append(f'printf("%zu\\n", sizeof({cname}));')
