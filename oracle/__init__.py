@@ -353,3 +353,87 @@ def troneu(b, coef, b1o=None):
     L.orc_troneu(ctypes.byref(g), _d(xx), _d(uf), _d(vf), _d(ff), _d(b.u1o), _d(b.uzo), _d(b.v1e), _d(b.vze),
                  _d(b1), _d(z))
     return z, b1
+
+
+# ---- 2-D bottom-pressure solve, CG2D4NONHY (nh_oracle_bot.c) --------------------------------------
+
+
+def _bot_lib():
+    L = lib()
+    if not getattr(L, "_bot", False):
+        from orctm_amd.abi import GridC
+
+        L.orc_bottom_rhs.argtypes = [ctypes.POINTER(_VGrid)] + [_dp] * 11
+        L.orc_op2d_coef.argtypes = [ctypes.POINTER(GridC), _dp]
+        L.orc_op2d_apply.argtypes = [ctypes.POINTER(GridC), _dp, _dp, _dp]
+        L.orc_solve2d.argtypes = [ctypes.POINTER(GridC), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_solve2d.restype = ctypes.c_int
+        L.orc_bottom_correction.argtypes = [ctypes.POINTER(GridC), _dp, _dp]
+        L._bot = True
+    return L
+
+
+def bottom_rhs(grid, v, divg, botgu=None, botgv=None, botgw=None):
+    """ocvtot_nh.f90:64-144: (DIVG_bot_surf (IE,JE) on the interior, BOTGU, BOTGV, BOTGW). The BOT* arrays
+    are the module arrays the source keeps between steps (zero if not given)."""
+    L = _bot_lib()
+    g = _vgrid(grid, v)
+    IE, JE = v.IE, v.JE
+    bu = np.zeros((IE - v.i_start + 1, JE), order="F") if botgu is None else np.array(botgu, order="F", copy=True)
+    bv = np.zeros((IE, JE - v.j_start + 1), order="F") if botgv is None else np.array(botgv, order="F", copy=True)
+    bw = np.zeros((IE, JE), order="F") if botgw is None else np.array(botgw, order="F", copy=True)
+    dbs = np.zeros((IE, JE), order="F")
+    dv = np.asfortranarray(divg, dtype=np.float64)
+    L.orc_bottom_rhs(ctypes.byref(g), _d(v.uoo), _d(v.uoo_1), _d(v.voe), _d(v.voe_1), _d(v.wno), _d(v.wno_1),
+                     _d(dv), _d(bu), _d(bv), _d(bw), _d(dbs))
+    return dbs, bu, bv, bw
+
+
+def op2d_coef(grid):
+    """compute_matrix_2d (mo_petsc.f90:401-538, Neumann off): 5 planes (centre, west, east, north, south)
+    over the (IE-2) x (JE-2) interior box, DMDA order."""
+    L = _bot_lib()
+    c = np.zeros((5, grid.nx * grid.ny))
+    L.orc_op2d_coef(ctypes.byref(grid.c_struct()), _d(c))
+    return c
+
+
+def op2d_apply(grid, coef, p):
+    L = _bot_lib()
+    y = np.zeros(grid.nx * grid.ny)
+    L.orc_op2d_apply(ctypes.byref(grid.c_struct()), _d(np.ascontiguousarray(coef)), _d(np.ascontiguousarray(p)), _d(y))
+    return y
+
+
+def op2d_scipy(grid, coef):
+    import scipy.sparse as sp
+
+    nx, ny = grid.nx, grid.ny
+    n = nx * ny
+    idx = np.arange(n)
+    i, j = idx % nx, idx // nx
+    rows, cols, vals = [idx], [idx], [coef[0]]
+    for plane, di, dj in ((1, -1, 0), (2, 1, 0), (3, 0, -1), (4, 0, 1)):
+        ok = (i + di >= 0) & (i + di < nx) & (j + dj >= 0) & (j + dj < ny) & (coef[plane] != 0.0)
+        rows.append(idx[ok])
+        cols.append(idx[ok] + di + dj * nx)
+        vals.append(coef[plane][ok])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def solve2d(grid, coef, b, x0=None, rtol=1e-10, maxit=100000):
+    """The device's 2-D PCG restated on the CPU: (x, iterations, rel_res)."""
+    L = _bot_lib()
+    x = np.zeros(grid.nx * grid.ny) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    rr = ctypes.c_double()
+    it = L.orc_solve2d(ctypes.byref(grid.c_struct()), _d(np.ascontiguousarray(coef)), _d(np.ascontiguousarray(b)),
+                       _d(x), rtol, maxit, ctypes.byref(rr))
+    return x, it, rr.value
+
+
+def bottom_correction(grid, pnh_bot, divg):
+    """ComputeRHS's CG2D4NONHY corrections (mo_petsc.f90:252-374) applied to a copy of DIVG."""
+    L = _bot_lib()
+    out = np.array(divg, order="F", dtype=np.float64, copy=True)
+    L.orc_bottom_correction(ctypes.byref(grid.c_struct()), _d(np.asfortranarray(pnh_bot, dtype=np.float64)), _d(out))
+    return out

@@ -1,0 +1,240 @@
+"""Pins the oracle's restatement of the 2-D bottom-pressure solve (CG2D4NONHY; oracle/nh_oracle_bot.c) by
+what the mathematics and the mesh fix:
+
+* compute_matrix_2d (mo_petsc.f90:401-538): the 5-point Laplacian in closed form on a uniform grid,
+  zero row sums inside the box, diagonal-only rows on its edge, and symmetry of S*A (S = DLXP*DLYP) on a
+  grid stretched along i, which a swapped west/east or north/south coupling would break;
+* the 2-D solve: SciPy's direct solve of the same matrix, a manufactured solution, the box-edge rows
+  exactly b / A_cc, and zero iterations from the exact solution;
+* the bottom values (ocvtot_nh.f90:64-144): the tendency at the deepest wet level of each column (an
+  independent formulation for masks wet from the top down, with the source's quirk that a column wet
+  down to KE-1 is never assigned), the literal "last assignment wins" order on a crafted column, the
+  values kept where the source assigns nothing, and the telescoping sum of the bottom divergence;
+* the RHS corrections (mo_petsc.f90:252-374): none for PNH_bot = 0 and for wet cells with wet
+  neighbours above a wet cell, linearity in PNH_bot, the lateral terms equal to the couplings the 3-D
+  operator holds for the same faces at wet levels, and the bottom-face term in closed form for a
+  flat-bottomed column.
+"""
+import copy
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+import oracle
+from cases import SPECS, build
+from orctm_amd import synth
+
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        g, divg = build(name)
+        _cache[name] = (g, synth.make_velocity(g))
+    return _cache[name]
+
+
+def test_op2d_uniform_closed_form():
+    g, v = case("cyclic_xy")  # dx = dy = 1850 m everywhere
+    c = oracle.op2d_coef(g)
+    nx, ny = g.nx, g.ny
+    dx, dy = g.spec.dx, g.spec.dy
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    inner = (i > 0) & (i < nx - 1) & (j > 0) & (j < ny - 1)
+    np.testing.assert_allclose(c[0], -(2 / dx**2 + 2 / dy**2), rtol=1e-15)
+    for p, h in ((1, dx), (2, dx), (3, dy), (4, dy)):
+        np.testing.assert_allclose(c[p][inner], 1 / h**2, rtol=1e-15)
+        assert np.all(c[p][~inner] == 0.0)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_op2d_row_sums_edges_and_symmetry(name):
+    g, v = case(name)
+    c = oracle.op2d_coef(g)
+    nx, ny = g.nx, g.ny
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    inner = (i > 0) & (i < nx - 1) & (j > 0) & (j < ny - 1)
+    assert np.all(c[0] < 0)
+    assert np.all(np.abs(c[1:].sum(axis=0) + c[0])[inner] <= 4e-16 * np.abs(c[0][inner]))
+    assert np.all(c[1:, ~inner] == 0.0)
+    # S*A restricted to the inner rows and columns is symmetric on these grids (DLYP constant along i,
+    # DLXP constant along j)
+    A = oracle.op2d_scipy(g, c).tocsr()
+    S = (g.dlxp[1:-1, 1:-1] * g.dlyp[1:-1, 1:-1]).ravel(order="F")
+    idx = np.flatnonzero(inner)
+    SA = (A.multiply(S[:, None])).tocsr()[idx][:, idx]
+    asym = abs(SA - SA.T).max()
+    assert asym <= 1e-14 * abs(SA).max()
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_solve2d_matches_direct_solve(name):
+    g, v = case(name)
+    d = oracle.divergence(g, v)
+    dbs, *_ = oracle.bottom_rhs(g, v, d)
+    c = oracle.op2d_coef(g)
+    b = dbs[1:-1, 1:-1].ravel(order="F")
+    x, it, rr = oracle.solve2d(g, c, b, rtol=1e-10)
+    assert rr <= 1e-10
+    xd = spl.spsolve(oracle.op2d_scipy(g, c).tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    nx, ny = g.nx, g.ny
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    edge = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1)
+    assert np.array_equal(x[edge], b[edge] / c[0][edge])
+    # from the exact solution nothing is left to do
+    x2, it2, rr2 = oracle.solve2d(g, c, b, x0=xd, rtol=1e-8)
+    assert it2 == 0 and rr2 <= 1e-8
+
+
+def test_solve2d_manufactured_solution():
+    g, v = case("stretched_dx")
+    c = oracle.op2d_coef(g)
+    nx, ny = g.nx, g.ny
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    ps = np.sin(0.3 * i + 0.1) * np.cos(0.7 * j) + 0.01 * i
+    b = oracle.op2d_apply(g, c, ps)
+    x, it, rr = oracle.solve2d(g, c, b, rtol=1e-12)
+    assert rr <= 1e-12
+    assert np.abs(x - ps).max() <= 1e-8 * np.abs(ps).max()
+
+
+def _gu(v, k):
+    return (v.uoo[:, :, k] - v.uoo_1[:, :, k]) / v.dt
+
+
+@pytest.mark.parametrize("name", ["regional", "closed", "stretched_dx"])
+def test_bottom_values_at_the_deepest_wet_level(name):
+    g, v = case(name)
+    d = oracle.divergence(g, v)
+    dbs, bu, bv, bw = oracle.bottom_rhs(g, v, d)
+    for mask, vel, vel1, got in ((v.amsuo, v.uoo, v.uoo_1, bu), (v.amsue, v.voe, v.voe_1, bv),
+                                 (v.weto, v.wno, v.wno_1, bw)):
+        wet = mask != 0.0
+        nwet = wet.sum(axis=2)
+        # masks here are wet from the top down: the wet levels are 0..nwet-1
+        assert np.array_equal(wet, np.arange(g.KE)[None, None, :] < nwet[:, :, None])
+        kb = np.maximum(nwet - 1, 0)
+        gk = (np.take_along_axis(vel[:, :, :g.KE], kb[:, :, None], axis=2)[:, :, 0]
+              - np.take_along_axis(vel1[:, :, :g.KE], kb[:, :, None], axis=2)[:, :, 0]) / v.dt
+        if got is bw:
+            with np.errstate(invalid="ignore", divide="ignore"):  # dry columns (DDPO 0) are masked below
+                gk = gk / np.take_along_axis(v.ddpo, kb[:, :, None], axis=2)[:, :, 0]
+        # The source's K = KE branch only looks at level KE, so a column whose last wet level is KE-1
+        # never meets its wet-to-dry step and keeps the value it had (zero here).
+        want = np.where((nwet > 0) & (nwet != g.KE - 1), gk, 0.0)
+        assert np.array_equal(got, want)
+        assert np.any(nwet == g.KE - 1) or name != "regional"
+
+
+def test_bottom_values_last_assignment_wins_and_kept_values():
+    g, v0 = case("closed")
+    v = copy.copy(v0)
+    am = v0.amsuo.copy(order="F")
+    # a column dry at the top and wet below: the K = 1 zero is overwritten at K = KE
+    i, j = 5, 6
+    am[i, j, :] = 1.0
+    am[i, j, 0] = 0.0
+    v.amsuo = am
+    d = oracle.divergence(g, v0)
+    bu_in = np.full((g.IE - g.i_start + 1, g.JE), 7.0, order="F")
+    dbs, bu, bv, bw = oracle.bottom_rhs(g, v, d, botgu=bu_in)
+    assert bu[i, j] == _gu(v, g.KE - 1)[i, j]
+    # one layer: a wet column gets no assignment and keeps its value, a dry one gets 0
+    g1, v1 = case("one_layer")
+    d1 = oracle.divergence(g1, v1)
+    bu_in = np.full((g1.IE - g1.i_start + 1, g1.JE), 7.0, order="F")
+    _, bu1, _, _ = oracle.bottom_rhs(g1, v1, d1, botgu=bu_in)
+    assert np.array_equal(bu1, np.where(v1.amsuo[:, :, 0] != 0.0, 7.0, 0.0))
+
+
+def test_bottom_divergence_telescopes():
+    g, v0 = case("regional")
+    v = copy.copy(v0)
+    v.voe_1 = v0.voe  # GV = 0
+    v.wno_1 = v0.wno  # GW = 0
+    zero = np.zeros((g.IE, g.JE, g.KE), order="F")
+    dbs, bu, bv, bw = oracle.bottom_rhs(g, v, zero)
+    assert np.all(bv == 0.0) and np.all(bw == 0.0)
+    IS = g.i_start
+    tel = (dbs[1:-1, 1:-1] * g.dlxp[1:-1, 1:-1]).sum(axis=0)
+    want = bu[g.IE - 1 - IS, 1:-1] - bu[1 - IS, 1:-1]  # BOTGU(IE-1) - BOTGU(1)
+    assert np.allclose(tel, want, rtol=0, atol=1e-12 * np.abs(bu).max())
+
+
+def test_correction_vanishes_for_zero_pnh_bot():
+    g, v = case("regional")
+    d = oracle.divergence(g, v)
+    out = oracle.bottom_correction(g, np.zeros((g.IE, g.JE), order="F"), d)
+    assert np.all(out == d)
+
+
+def test_correction_linearity_and_untouched_wet_interior():
+    g, v = case("regional")
+    d = oracle.divergence(g, v)
+    p1 = np.asfortranarray(synth.normal(5, 1, g.IE * g.JE).reshape(g.JE, g.IE).T)
+    p2 = np.asfortranarray(synth.normal(6, 1, g.IE * g.JE).reshape(g.JE, g.IE).T)
+    c1 = oracle.bottom_correction(g, p1, d) - d
+    c2 = oracle.bottom_correction(g, p2, d) - d
+    c12 = oracle.bottom_correction(g, p1 + 2 * p2, d) - d
+    assert np.allclose(c12, c1 + 2 * c2, rtol=1e-9, atol=1e-12 * np.abs(c12).max())
+    # wet cells whose four lateral neighbours and the cell below are wet get no correction
+    w = g.weto
+    I = slice(1, -1)
+    inner = np.zeros_like(w, dtype=bool)
+    below = np.concatenate([w[:, :, 1:], np.zeros_like(w[:, :, :1])], axis=2) != 0
+    above = np.concatenate([np.ones_like(w[:, :, :1]), w[:, :, :-1]], axis=2) != 0
+    inner[I, I, :] = ((w[I, I] != 0) & (w[:-2, I] != 0) & (w[2:, I] != 0) & (w[I, :-2] != 0) & (w[I, 2:] != 0)
+                      & below[I, I] & above[I, I])
+    assert inner.sum() > 100
+    out = oracle.bottom_correction(g, p1, d)
+    assert np.array_equal(out[inner], d[inner])
+
+
+def test_correction_lateral_terms_are_the_3d_couplings():
+    """PNH_bot nonzero in one column only: a wet neighbour cell whose face to that column is dry is
+    corrected by -(coupling) * PNH_bot, where the coupling is the one compute_matrix inserts for the same
+    face at a level where both cells are wet."""
+    g, v = case("regional")
+    A, S = oracle.assemble(g)
+    nx, ny = g.nx, g.ny
+    w = g.weto
+    d = np.zeros((g.IE, g.JE, g.KE), order="F")
+    checked = 0
+    # neighbour offset (di, dj) -> plane of the 3-D operator seen from the neighbour towards the column
+    planes = {(1, 0): 1, (-1, 0): 2, (0, 1): 3, (0, -1): 4}
+    for (I0, J0) in [(a, b) for a in range(3, g.IE - 3, 7) for b in range(3, g.JE - 3, 5)]:
+        p = np.zeros((g.IE, g.JE), order="F")
+        p[I0, J0] = 0.75
+        corr = oracle.bottom_correction(g, p, d)
+        for (di, dj), plane in planes.items():
+            I, J = I0 + di, J0 + dj
+            both = np.flatnonzero((w[I, J] != 0) & (w[I0, J0] != 0))
+            if both.size == 0:
+                continue
+            n = (I - 1) + nx * ((J - 1) + ny * both[0])
+            coup = A[plane, n]
+            for k in range(g.KE):
+                if w[I, J, k] != 0 and w[I0, J0, k] == 0:
+                    assert corr[I, J, k] == -(coup * 0.75)
+                    checked += 1
+    assert checked > 5
+
+
+def test_correction_bottom_face_closed_form():
+    g, v = case("cyclic_xy")  # flat bottom: every column wet down to KE (the last cell partial)
+    assert np.all(g.weto != 0)
+    p = np.zeros((g.IE, g.JE), order="F")
+    I0, J0 = 10, 8
+    p[I0, J0] = 2.0
+    d = np.zeros((g.IE, g.JE, g.KE), order="F")
+    corr = oracle.bottom_correction(g, p, d)
+    # the bottom face: ZW(KEP) = 1/a with a = DZW(KE) - DDPO(KE)/2 (ComputeRHS's own KEP rule, not
+    # compute_matrix's DDPO(KE)); ZW(KE) = 1/c with c = (DDPO(KE-1) + DDPO(KE))/2; ZP(KE) = 2/(c + a);
+    # Z_D = ZW(KEP) ZP(KE) = 2 / (a (a + c))
+    dd = g.ddpo[I0, J0]
+    a = g.dzw[-1] - dd[-1] / 2
+    c = (dd[-2] + dd[-1]) / 2
+    assert np.isclose(corr[I0, J0, -1], -2.0 / (a * (a + c)) * 2.0, rtol=1e-14)
+    assert np.count_nonzero(corr) == 1
