@@ -95,7 +95,11 @@ typedef struct orctm_nh_options {
   int32_t tile_rows;    /* rows (j) per tile of the inner-iteration kernels: 0 (default) = tuned per size
                            class at create / batch create; 4, 8 or 12 = fixed. Results are the same bits
                            for every choice (block sums are formed over fixed 4-row groups). */
-  int32_t reserved[4];
+  int32_t bottom_solve; /* 1: ORCTM's CG2D4NONHY option in orctm_nh_divergence: the 2-D bottom-pressure solve
+                           and the corrections of DIVG by PNH_bot before the right-hand side is formed
+                           (ocvtot_nh.f90:64-147, mo_petsc.f90:147-208, 252-374, 401-538). Default 0, as
+                           the shipped Makefile builds (-DZZCG2D4NONHY). */
+  int32_t reserved[3];
 } orctm_nh_options_t;
 
 /* Per-solve statistics. */
@@ -238,6 +242,13 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
  * exchanged. pnh, uoo, voe are read and written, wno (IE,JE,KEP) is written. The copies UOO_1 = UOO,
  * VOE_1 = VOE, WNO_1 = WNO, WO = WNO (:232-249) stay with the caller. */
 int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, double* wno);
+/* With options.bottom_solve: the state of the last 2-D bottom solve (petsc_solve_2D, mo_petsc.f90:147-166,
+ * then bounds_exch(PNH_bot), ocvtot_nh.f90:146). pnh_bot (IE,JE), halo included (zero on a closed side,
+ * as the source leaves it); divg_bot_surf (IE,JE), interior written, halo untouched (the source's halo
+ * values are never read); iterations and rel_res (||b - A2 p||_2 / ||b||_2) of that solve. Any pointer
+ * may be NULL. The 2-D solve's status is also what orctm_nh_divergence returns when it is not OK. */
+int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, int32_t* iterations,
+                        double* rel_res);
 
 /* ------------------------------------------------------------------------------------------
  * The barotropic free-surface solve of the hydrostatic core: ITPREP (ORCTM/src_orctm/itprep.f90:15-38,

@@ -35,7 +35,7 @@ class OptionsC(ctypes.Structure):
         ("precision", ctypes.c_int32), ("maxit", ctypes.c_int32), ("max_outer", ctypes.c_int32),
         ("warm_start", ctypes.c_int32), ("rtol", ctypes.c_double), ("rtol_inner", ctypes.c_double),
         ("check_every", ctypes.c_int32), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("tile_rows", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("tile_rows", ctypes.c_int32), ("bottom_solve", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -109,7 +109,7 @@ EXPORTS = [
     "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
     "orctm_nh_server_destroy", "orctm_nh_attach_vgrid", "orctm_nh_divergence", "orctm_nh_project",
     "orctm_nh_set_profile", "orctm_nh_sor_create", "orctm_nh_troneu", "orctm_nh_sor_info",
-    "orctm_nh_sor_destroy",
+    "orctm_nh_sor_destroy", "orctm_nh_get_bottom",
 ]
 
 _lib = None

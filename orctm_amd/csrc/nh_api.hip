@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/orctm_nh.h"
+#include "nh_bot.h"
 #include "nh_kernels.h"
 #include "nh_lock.h"
 #include "nh_ocv.h"
@@ -117,6 +118,7 @@ struct orctm_nh {
   OcvArrays ocv{};                 // metrics attached by orctm_nh_attach_vgrid + velocity buffers
   double *d_uoo = nullptr, *d_uoo_1 = nullptr, *d_voe = nullptr, *d_voe_1 = nullptr;
   double *d_wno = nullptr, *d_wno_1 = nullptr, *d_zo = nullptr;
+  std::unique_ptr<BotState> bot;   // the 2-D bottom-pressure solve (options.bottom_solve)
   std::mutex mu;
 };
 
@@ -218,6 +220,8 @@ hipError_t ensure_fort(orctm_nh_t* h) {
 }
 
 void free_all(orctm_nh_t* h) {
+  if (h->bot) bot_release(*h->bot);
+  h->bot.reset();
   for (void* p : h->allocs) hipFree(p);
   h->allocs.clear();
 }
@@ -705,6 +709,65 @@ bool grid_valid(const orctm_nh_grid_t* g) {
 
 hipError_t upload(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }
 
+// The 2-D bottom-pressure solve's device state (options.bottom_solve): the metrics it needs from the
+// grid, the zeroed module arrays (beleg_zero.f90:481-504), the work vectors and the 2-D operator.
+hipError_t bot_create(orctm_nh_t* h, const orctm_nh_grid_t* g) {
+  auto b = std::make_unique<BotState>();
+  b->IE = h->IE;
+  b->JE = h->JE;
+  b->KE = h->KE;
+  b->IS = h->i_start;
+  b->JS = h->j_start;
+  b->cyc_x = h->cyc_x;
+  b->cyc_y = h->cyc_y;
+  b->nx = h->nx;
+  b->ny = h->ny;
+  const size_t IE = h->IE, JE = h->JE;
+  const size_t p2 = IE * JE, u2 = (IE - h->i_start + 1) * JE, v2 = IE * (JE - h->j_start + 1);
+  const size_t n = (size_t)h->nx * h->ny, nblk = (n + 255) / 256;
+  hipError_t e = hipSuccess;
+  auto buf = [&](double*& dst, size_t cnt) {
+    void* p = nullptr;
+    if (e == hipSuccess) e = dalloc(h, &p, cnt * sizeof(double));
+    dst = static_cast<double*>(p);
+  };
+  auto put = [&](const double*& dst, const double* src, size_t cnt) {
+    double* p = nullptr;
+    buf(p, cnt);
+    if (e == hipSuccess) e = upload(p, src, cnt * sizeof(double));
+    dst = p;
+  };
+  put(b->dlxu, g->dlxu, u2);
+  put(b->dlyv, g->dlyv, v2);
+  put(b->dlxp, g->dlxp, p2);
+  put(b->dlyp, g->dlyp, p2);
+  put(b->dzw, g->dzw, h->KE);
+  buf(b->coef, 5 * n);
+  buf(b->S, n);
+  buf(b->botgu, u2);
+  buf(b->botgv, v2);
+  buf(b->botgw, p2);
+  buf(b->dbs, p2);
+  buf(b->pnh_bot, p2);
+  buf(b->b, n);
+  buf(b->x, n);
+  buf(b->r, n);
+  buf(b->z, n);
+  buf(b->d0, n);
+  buf(b->d1, n);
+  buf(b->q, n);
+  buf(b->part, 3 * nblk);
+  buf(b->scal, 8);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->scal_host, 8 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = bot_setup(h->ctx.stream, *b);
+  if (e != hipSuccess) {
+    bot_release(*b);
+    return e;
+  }
+  h->bot = std::move(b);
+  return hipSuccess;
+}
+
 // Device part of create: allocations, assembly, factorisation, single-system context.
 int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   const size_t NP = (size_t)h->plane * h->nz;
@@ -914,6 +977,7 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   if (o.check_every <= 0) o.check_every = 16;
   if (!(o.rtol > 0)) o.rtol = 1e-10;
   if (o.tile_rows != 0 && o.tile_rows != 4 && o.tile_rows != 8 && o.tile_rows != 12) return ORCTM_NH_ERR_INVALID;
+  if (o.bottom_solve != 0 && o.bottom_solve != 1) return ORCTM_NH_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device || o.device < 0) return ORCTM_NH_ERR_NODEVICE;
   if (hipSetDevice(o.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
@@ -943,6 +1007,7 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   h->plane = (long long)h->pitch * h->ny;
   h->fp32 = (o.precision == ORCTM_NH_MIXED);
   int rc = create_device(h, grid);
+  if (rc == ORCTM_NH_OK && o.bottom_solve) rc = map_err(bot_create(h, grid));
   if (rc != ORCTM_NH_OK) {
     ctx_destroy(h->ctx);
     free_all(h);
@@ -1343,8 +1408,15 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_wno, wno, w3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_wno_1, wno_1, w3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_zo, zo, IE * JE * 8, hipMemcpyHostToDevice, st);
-  // DIVG into the Fortran staging array, then b = DIVG*WETO exactly as orctm_nh_set_rhs forms it
+  // DIVG into the Fortran staging array; with the bottom solve, PNH_bot and the corrections of DIVG
+  // (ComputeRHS with CG2D4NONHY); then b = DIVG*WETO exactly as orctm_nh_set_rhs forms it
   if (e == hipSuccess) e = launch_divergence(st, h->ocv, h->d_fort);
+  int bot_status = ORCTM_NH_OK;
+  if (e == hipSuccess && h->bot) {
+    e = bot_step(st, *h->bot, h->ocv, h->d_fort, h->opts.rtol, h->opts.maxit);
+    if (e == hipSuccess) e = bot_correct(st, *h->bot, h->ocv, h->d_fort);
+    if (e == hipSuccess) bot_status = h->bot->status;
+  }
   if (e == hipSuccess)
     e = launch_rhs(st, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, h->d_fort, h->IE, h->JE);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1354,6 +1426,24 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
       e = hipMemcpy2D(divg + off, IE * 8, h->d_fort + off, IE * 8, (IE - 2) * 8, JE - 2, hipMemcpyDeviceToHost);
     }
   }
+  return e == hipSuccess ? bot_status : map_err(e);
+}
+
+int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, int32_t* iterations,
+                        double* rel_res) {
+  if (!h) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DEVLOCK;
+  if (!h->bot) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(h->opts.device);
+  const size_t IE = h->IE, JE = h->JE;
+  hipError_t e = hipSuccess;
+  if (pnh_bot) e = hipMemcpy(pnh_bot, h->bot->pnh_bot, IE * JE * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && divg_bot_surf)
+    e = hipMemcpy2D(divg_bot_surf + 1 + IE, IE * 8, h->bot->dbs + 1 + IE, IE * 8, (IE - 2) * 8, JE - 2,
+                    hipMemcpyDeviceToHost);
+  if (iterations) *iterations = h->bot->iters < 0 ? 0 : h->bot->iters;
+  if (rel_res) *rel_res = h->bot->rel_res;
   return map_err(e);
 }
 

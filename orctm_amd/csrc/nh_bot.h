@@ -1,0 +1,43 @@
+// nh_bot.h -- the 2-D bottom-pressure solve of the CG2D4NONHY option on the device (nh_bot.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nh_ocv.h"
+
+namespace onh {
+
+// Device state of the bottom solve of one handle. Arrays in the caller's Fortran layouts with halos,
+// except the box vectors, which hold the (IE-2) x (JE-2) interior box in DMDA order (i fastest).
+struct BotState {
+  int IE = 0, JE = 0, KE = 0, IS = 1, JS = 1, cyc_x = 0, cyc_y = 0, nx = 0, ny = 0;
+  const double *dlxu = nullptr, *dlyv = nullptr, *dlxp = nullptr, *dlyp = nullptr, *dzw = nullptr;
+  double* coef = nullptr;  // 5 planes of nx*ny: centre, west, east, north, south of compute_matrix_2d
+  double* S = nullptr;     // DLXP*DLYP per box cell
+  double *botgu = nullptr, *botgv = nullptr, *botgw = nullptr;  // module arrays kept between steps
+  double *dbs = nullptr, *pnh_bot = nullptr;                      // DIVG_bot_surf, PNH_bot (IE,JE)
+  double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
+  double* part = nullptr;  // block partial sums, 3 per block
+  double* scal = nullptr;  // rz, dq, alpha, beta, rr, bb, breakdown
+  double* scal_host = nullptr;  // pinned copy of scal
+  int nblk = 0;
+  hipGraphExec_t chunk = nullptr;  // kBotChunk iterations
+  int iters = -1;                  // iterations of the last solve, -1 before the first
+  double rel_res = 0.0;
+  int status = 0;
+};
+
+constexpr int kBotChunk = 32;
+
+// coef and S from the metrics (mo_petsc.f90:401-538). The caller allocates every array of BotState
+// (zero-filled: the module arrays start at 0 as beleg_zero.f90:481-504 sets them) and scal_host pinned.
+hipError_t bot_setup(hipStream_t st, BotState& b);
+// ocvtot_nh.f90:64-147 and petsc_solve_2D (mo_petsc.f90:147-166): BOTGU/BOTGV/BOTGW, DIVG_bot_surf from
+// the velocity state and DIVG (first level, interior) of this step, then PNH_bot from the 2-D solve
+// (warm start from the previous solution) and its bounds_exch. b.status, b.iters, b.rel_res report it.
+hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const double* divg, double rtol, int maxit);
+// ComputeRHS with CG2D4NONHY (mo_petsc.f90:252-374): DIVG corrected in place on the interior.
+hipError_t bot_correct(hipStream_t st, const BotState& b, const OcvArrays& a, double* divg);
+void bot_release(BotState& b);
+
+}  // namespace onh

@@ -32,7 +32,7 @@ MODULE orctm_nh_c
     INTEGER(C_INT32_T) :: precision, maxit, max_outer, warm_start
     REAL(C_DOUBLE) :: rtol, rtol_inner
     INTEGER(C_INT32_T) :: check_every, device, profile, tile_rows
-    INTEGER(C_INT32_T) :: reserved(4)
+    INTEGER(C_INT32_T) :: bottom_solve, reserved(3)
   END TYPE
 
   TYPE, BIND(C) :: orctm_nh_stats_t

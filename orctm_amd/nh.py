@@ -36,8 +36,10 @@ def _dp(a: np.ndarray):
 
 def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optional[float] = None,
             maxit: int = 10000, max_outer: int = 40, warm_start: bool = True, check_every: int = 16,
-            device: int = 0, profile: bool = False, tile_rows: int = 0) -> abi.OptionsC:
-    """Solver options (orctm_nh_options_t). tile_rows: 0 = tuned per size class, or 4, 8, 12."""
+            device: int = 0, profile: bool = False, tile_rows: int = 0, bottom_solve: bool = False) -> abi.OptionsC:
+    """Solver options (orctm_nh_options_t). tile_rows: 0 = tuned per size class, or 4, 8, 12.
+    bottom_solve: ORCTM's CG2D4NONHY option in divergence() (the 2-D bottom-pressure solve and the
+    corrections of DIVG by PNH_bot)."""
     lib = abi.load()
     o = abi.OptionsC()
     lib.orctm_nh_default_options(ctypes.byref(o))
@@ -51,6 +53,7 @@ def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optiona
     o.device = device
     o.profile = 1 if profile else 0
     o.tile_rows = tile_rows
+    o.bottom_solve = 1 if bottom_solve else 0
     return o
 
 
@@ -110,7 +113,9 @@ class NHSolver:
         self._rc(self.lib.orctm_nh_attach_vgrid(self.h, ctypes.byref(vg)), "orctm_nh_attach_vgrid")
 
     def divergence(self, uoo, uoo_1, voe, voe_1, wno, wno_1, zo, divg: Optional[np.ndarray] = None) -> None:
-        """DIVG from the velocity tendencies (ocvtot_nh.f90:11-62); the resident b becomes DIVG*WETO."""
+        """DIVG from the velocity tendencies (ocvtot_nh.f90:11-62); the resident b becomes DIVG*WETO. With
+        bottom_solve, PNH_bot is solved first and DIVG corrected by it (CG2D4NONHY); `divg` then receives
+        the corrected DIVG, as the source's ComputeRHS leaves it."""
         arrs = (uoo, uoo_1, voe, voe_1, wno, wno_1, zo)
         for a, kind, n in zip(arrs, ("u3", "u3", "v3", "v3", "w3", "w3", "p2"),
                               ("uoo", "uoo_1", "voe", "voe_1", "wno", "wno_1", "zo")):
@@ -119,6 +124,17 @@ class NHSolver:
             self._check_f(divg, "divg")
         self._rc(self.lib.orctm_nh_divergence(self.h, *[_dp(a) for a in arrs],
                                               _dp(divg) if divg is not None else None), "orctm_nh_divergence")
+
+    def get_bottom(self) -> dict:
+        """With bottom_solve: PNH_bot (IE,JE, halo included), DIVG_bot_surf (IE,JE, interior), and the
+        iterations and relative residual of the last 2-D bottom solve (orctm_nh_get_bottom)."""
+        IE, JE, KE = self.shape
+        p = np.zeros((IE, JE), order="F")
+        d = np.zeros((IE, JE), order="F")
+        it, rr = ctypes.c_int32(), ctypes.c_double()
+        self._rc(self.lib.orctm_nh_get_bottom(self.h, _dp(p), _dp(d), ctypes.byref(it), ctypes.byref(rr)),
+                 "orctm_nh_get_bottom")
+        return {"pnh_bot": p, "divg_bot_surf": d, "iterations": it.value, "rel_res": rr.value}
 
     def project(self, pnh, uoo, voe, wno) -> None:
         """PNH halo, U/V pressure correction and WNO by continuity (ocvtot_nh.f90:151-230), in place."""

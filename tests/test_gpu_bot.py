@@ -1,0 +1,118 @@
+"""GPU parity for the 2-D bottom-pressure solve (CG2D4NONHY: ocvtot_nh.f90:64-147, mo_petsc.f90:147-208,
+252-374, 401-538) run inside orctm_nh_divergence when options.bottom_solve is set, through the C ABI.
+
+Bar:
+* DIVG_bot_surf bit-exact against the oracle (same operations in the same order), also on a second step
+  where the module arrays BOTGU/BOTGV/BOTGW carry over from the first;
+* PNH_bot within 1e-8 (relative, 2-norm) of SciPy's direct solve of compute_matrix_2d's matrix with the
+  oracle's right-hand side, its residual recomputed with the oracle's operator <= 1e-10 relative, the
+  box-edge rows exactly b / A_cc, the halo as bounds_exch leaves it;
+* the corrected DIVG bit-exact against the oracle's corrections fed the device's PNH_bot, and the 3-D
+  solve from it bit-identical to a solve from that DIVG handed over by the host;
+* the warm start: the same step again needs no 2-D iterations.
+"""
+import copy
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+import oracle
+from cases import SPECS, build
+from orctm_amd import abi, synth
+from orctm_amd.nh import NHSolver, options
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        g, divg = build(name)
+        _cache[name] = (g, synth.make_velocity(g))
+    return _cache[name]
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def step(s, v, divg=None):
+    s.divergence(v.uoo, v.uoo_1, v.voe, v.voe_1, v.wno, v.wno_1, v.zo, divg)
+    return s.get_bottom()
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_bottom_rhs_solve_and_correction(name):
+    g, v = case(name)
+    print(f"[bot] {name}", flush=True)
+    d = oracle.divergence(g, v)
+    dbs_ref, bu, bv, bw = oracle.bottom_rhs(g, v, d)
+    divg = np.zeros((g.IE, g.JE, g.KE), order="F")
+    with NHSolver(g, options(precision=abi.FP64, bottom_solve=True)) as s:
+        s.attach_vgrid(v)
+        out = step(s, v, divg)
+        st = s.solve_resident()
+        p_dev = s.get_solution(np.zeros((g.IE, g.JE, g.KE), order="F"))
+    assert bits_equal(out["divg_bot_surf"][1:-1, 1:-1], dbs_ref[1:-1, 1:-1])
+    # the 2-D solve
+    c = oracle.op2d_coef(g)
+    b = dbs_ref[1:-1, 1:-1].ravel(order="F")
+    pb = out["pnh_bot"]
+    x = pb[1:-1, 1:-1].ravel(order="F")
+    xd = spl.spsolve(oracle.op2d_scipy(g, c).tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    rr = np.linalg.norm(b - oracle.op2d_apply(g, c, x)) / np.linalg.norm(b)
+    assert rr <= 1e-10 and out["rel_res"] <= 1e-10
+    nx, ny = g.nx, g.ny
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    edge = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1)
+    assert np.array_equal(x[edge], b[edge] / c[0][edge])
+    halo = np.zeros((g.IE, g.JE), order="F")
+    halo[1:-1, 1:-1] = pb[1:-1, 1:-1]
+    oracle._ocv_lib().orc_bounds_exch(oracle._d(halo), g.IE, g.JE, 1, 1, 1, int(g.spec.cyclic_x),
+                                      int(g.spec.cyclic_y))
+    assert bits_equal(pb, halo)
+    # the corrections of DIVG, fed the device's PNH_bot, and the 3-D solve from the corrected DIVG
+    dc = oracle.bottom_correction(g, pb, d)
+    assert bits_equal(divg[1:-1, 1:-1, :], dc[1:-1, 1:-1, :])
+    with NHSolver(g, options(precision=abi.FP64)) as t:
+        t.set_rhs(np.asfortranarray(dc))
+        st2 = t.solve_resident()
+        p2 = t.get_solution(np.zeros((g.IE, g.JE, g.KE), order="F"))
+    assert st["rc"] == abi.OK and st2["rc"] == abi.OK
+    assert bits_equal(p_dev, p2)
+
+
+def test_bottom_state_carries_over_and_warm_start():
+    g, v1 = case("regional")
+    v2 = copy.copy(v1)
+    rng = np.random.default_rng(3)
+    v2.uoo = np.asfortranarray(v1.uoo * (1.0 + 0.01 * rng.standard_normal(v1.uoo.shape)))
+    v2.voe = np.asfortranarray(v1.voe * (1.0 + 0.01 * rng.standard_normal(v1.voe.shape)))
+    v2.wno = np.asfortranarray(v1.wno * (1.0 + 0.01 * rng.standard_normal(v1.wno.shape)))
+    d1 = oracle.divergence(g, v1)
+    _, bu, bv, bw = oracle.bottom_rhs(g, v1, d1)
+    d2 = oracle.divergence(g, v2)
+    dbs2, *_ = oracle.bottom_rhs(g, v2, d2, botgu=bu, botgv=bv, botgw=bw)
+    with NHSolver(g, options(precision=abi.MIXED, bottom_solve=True)) as s:
+        s.attach_vgrid(v1)
+        o1 = step(s, v1)
+        o1b = step(s, v1)  # the same step again: warm-started at the solution
+        o2 = step(s, v2)
+    assert o1["iterations"] > 0 and o1b["iterations"] == 0
+    assert bits_equal(o1b["pnh_bot"], o1["pnh_bot"])
+    assert bits_equal(o2["divg_bot_surf"][1:-1, 1:-1], dbs2[1:-1, 1:-1])
+    assert 0 < o2["iterations"] <= o1["iterations"]
+    assert o2["rel_res"] <= 1e-10
+
+
+def test_get_bottom_needs_the_option():
+    g, v = case("closed")
+    with NHSolver(g, options(precision=abi.MIXED)) as s:
+        with pytest.raises(Exception) as ei:
+            s.get_bottom()
+        assert getattr(ei.value, "code", None) == abi.ERR_INVALID
