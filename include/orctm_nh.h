@@ -245,10 +245,11 @@ int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, doubl
 /* With options.bottom_solve: the state of the last 2-D bottom solve (petsc_solve_2D, mo_petsc.f90:147-166,
  * then bounds_exch(PNH_bot), ocvtot_nh.f90:146). pnh_bot (IE,JE), halo included (zero on a closed side,
  * as the source leaves it); divg_bot_surf (IE,JE), interior written, halo untouched (the source's halo
- * values are never read); iterations and rel_res (||b - A2 p||_2 / ||b||_2) of that solve. Any pointer
- * may be NULL. The 2-D solve's status is also what orctm_nh_divergence returns when it is not OK. */
+ * values are never read); iterations and rel_res (||b - A2 p||_2 / ||b||_2) of that solve, and its device
+ * time from the bottom values to the exchange of PNH_bot (HIP events). Any pointer may be NULL. The 2-D
+ * solve's status is also what orctm_nh_divergence returns when it is not OK. */
 int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, int32_t* iterations,
-                        double* rel_res);
+                        double* rel_res, double* device_ms);
 
 /* ------------------------------------------------------------------------------------------
  * The barotropic free-surface solve of the hydrostatic core: ITPREP (ORCTM/src_orctm/itprep.f90:15-38,

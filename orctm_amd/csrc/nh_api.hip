@@ -759,6 +759,8 @@ hipError_t bot_create(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   buf(b->part, 3 * nblk);
   buf(b->scal, 8);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->scal_host, 8 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e == hipSuccess) e = bot_setup(h->ctx.stream, *b);
   if (e != hipSuccess) {
     bot_release(*b);
@@ -1430,7 +1432,7 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
 }
 
 int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, int32_t* iterations,
-                        double* rel_res) {
+                        double* rel_res, double* device_ms) {
   if (!h) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
@@ -1444,6 +1446,12 @@ int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, i
                     hipMemcpyDeviceToHost);
   if (iterations) *iterations = h->bot->iters < 0 ? 0 : h->bot->iters;
   if (rel_res) *rel_res = h->bot->rel_res;
+  if (e == hipSuccess && device_ms) {
+    float ms = 0.f;
+    *device_ms = 0.0;
+    if (h->bot->iters >= 0) e = hipEventElapsedTime(&ms, h->bot->ev0, h->bot->ev1);
+    if (e == hipSuccess) *device_ms = ms;
+  }
   return map_err(e);
 }
 

@@ -22,6 +22,7 @@ struct BotState {
   double* scal_host = nullptr;  // pinned copy of scal
   int nblk = 0;
   hipGraphExec_t chunk = nullptr;  // kBotChunk iterations
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the last bot_step
   int iters = -1;                  // iterations of the last solve, -1 before the first
   double rel_res = 0.0;
   int status = 0;
@@ -30,7 +31,8 @@ struct BotState {
 constexpr int kBotChunk = 32;
 
 // coef and S from the metrics (mo_petsc.f90:401-538). The caller allocates every array of BotState
-// (zero-filled: the module arrays start at 0 as beleg_zero.f90:481-504 sets them) and scal_host pinned.
+// (zero-filled: the module arrays start at 0 as beleg_zero.f90:481-504 sets them), scal_host pinned
+// and the two events.
 hipError_t bot_setup(hipStream_t st, BotState& b);
 // ocvtot_nh.f90:64-147 and petsc_solve_2D (mo_petsc.f90:147-166): BOTGU/BOTGV/BOTGW, DIVG_bot_surf from
 // the velocity state and DIVG (first level, interior) of this step, then PNH_bot from the 2-D solve

@@ -438,6 +438,7 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
   const BDims g = dims(b);
   const int n = b.nx * b.ny;
   const long long ncol = (long long)g.NU * g.JE + (long long)g.IE * g.NV + (long long)g.IE * g.JE;
+  BCHK(hipEventRecord(b.ev0, st));
   hipLaunchKernelGGL(k_bot_extract, dim3(nblocks(ncol)), dim3(BB), 0, st, g, a, b.botgu, b.botgv, b.botgw);
   BCHK(hipGetLastError());
   hipLaunchKernelGGL(k_bot_rhs, dim3(nblocks(n)), dim3(BB), 0, st, g, a, (const double*)b.botgu,
@@ -489,7 +490,8 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
   b.rel_res = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
   hipLaunchKernelGGL(k2_scatter, dim3(nblocks(n)), dim3(BB), 0, st, g, (const double*)b.x, b.pnh_bot);
   BCHK(hipGetLastError());
-  return launch_exchange(st, b.pnh_bot, b.IE, b.JE, 1, 1, 1, b.cyc_x, b.cyc_y);
+  BCHK(launch_exchange(st, b.pnh_bot, b.IE, b.JE, 1, 1, 1, b.cyc_x, b.cyc_y));
+  return hipEventRecord(b.ev1, st);
 }
 
 hipError_t bot_correct(hipStream_t st, const BotState& b, const OcvArrays& a, double* divg) {
@@ -502,6 +504,9 @@ hipError_t bot_correct(hipStream_t st, const BotState& b, const OcvArrays& a, do
 void bot_release(BotState& b) {
   if (b.chunk) hipGraphExecDestroy(b.chunk);
   b.chunk = nullptr;
+  if (b.ev0) hipEventDestroy(b.ev0);
+  if (b.ev1) hipEventDestroy(b.ev1);
+  b.ev0 = b.ev1 = nullptr;
   if (b.scal_host) hipHostFree(b.scal_host);
   b.scal_host = nullptr;
 }

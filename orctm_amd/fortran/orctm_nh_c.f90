@@ -6,8 +6,9 @@
 !   petsc_end       (mo_petsc.f90:131-143, called from orctm.f90:1585,1651)
 !
 ! In ocvtot_nh.f90 and orctm.f90 replace `use mo_petsc` by `use orctm_nh_c`; nothing else changes.
-! Optionally, ocvtot_nh_device replaces the body of OCVTOT_NH (ocvtot_nh.f90:11-249, single process,
-! CG2D4NONHY off): the divergence, the solve and the projection then all run on the device; and
+! Optionally, ocvtot_nh_device replaces the body of OCVTOT_NH (ocvtot_nh.f90:11-249, single process):
+! the divergence, with CG2D4NONHY the 2-D bottom-pressure solve and the corrections of DIVG, the solve
+! and the projection then all run on the device; and
 ! troneu_device replaces CALL TRONEU (orctm.f90:1414): the barotropic SOR solve on the device.
 ! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
 ! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
@@ -80,6 +81,15 @@ MODULE orctm_nh_c
       REAL(C_DOUBLE), INTENT(IN) :: uoo(*), uoo_1(*), voe(*), voe_1(*), wno(*), wno_1(*), zo(*)
       REAL(C_DOUBLE), INTENT(INOUT) :: divg(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_get_bottom(handle, pnh_bot, divg_bot_surf, iterations, rel_res, &
+        device_ms) BIND(C, name="orctm_nh_get_bottom")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT32_T
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(OUT) :: pnh_bot(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: divg_bot_surf(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: iterations
+      REAL(C_DOUBLE), INTENT(OUT) :: rel_res, device_ms
+    END FUNCTION
     INTEGER(C_INT) FUNCTION orctm_nh_project(handle, pnh, uoo, voe, wno) BIND(C, name="orctm_nh_project")
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: handle
@@ -136,6 +146,9 @@ CONTAINS
     g%open_w = MERGE(1, 0, West);  g%open_e = MERGE(1, 0, East)
     g%open_n = MERGE(1, 0, North); g%open_s = MERGE(1, 0, South)
     CALL orctm_nh_default_options(o)
+#ifdef CG2D4NONHY
+    o%bottom_solve = 1
+#endif
     ierr = orctm_nh_create(g, o, nh_handle)
     IF (ierr /= ORCTM_NH_OK) THEN
       PRINT *, 'orctm_nh_create failed, status ', ierr
@@ -152,13 +165,20 @@ CONTAINS
     IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_solve status ', ierr, ' rel.res ', st%rel_res
   END SUBROUTINE petsc_solve_3D
 
-  ! The whole of OCVTOT_NH (ocvtot_nh.f90:1-259, single process, CG2D4NONHY off) on the device:
-  ! DIVG from the velocity tendencies (:11-62), the solve (:150), the PNH halo, the U/V correction
-  ! and WNO by continuity (:151-230); the copies of :232-249 stay here on the host.
+  ! The whole of OCVTOT_NH (ocvtot_nh.f90:1-259, single process) on the device: DIVG from the velocity
+  ! tendencies (:11-62); with CG2D4NONHY the bottom values, the 2-D solve for PNH_bot (:64-147) and the
+  ! corrections of DIVG (mo_petsc.f90:252-374), PNH_bot and DIVG_bot_surf copied back for the output
+  ! (wrte_mean.f90:202-207); the solve (:150), the PNH halo, the U/V correction and WNO by continuity
+  ! (:151-230); the copies of :232-249 stay here on the host.
   SUBROUTINE ocvtot_nh_device
     USE MO_PARAM1, ONLY: IE, JE, KE, KEP, I_start, J_start
     USE MO_COMMO1, ONLY: DLXP, DLYP, DDPO, WETO, AMSUO, DDUO, DTDXUO, DLYU, AMSUE, DDUE, DPYE, DLXV, &
                          DTDXPO, DTDYO, DT, DTI, UOO, UOO_1, VOE, VOE_1, WNO, WNO_1, WO, ZO, DIVG, PNH
+#ifdef CG2D4NONHY
+    USE MO_COMMO1, ONLY: PNH_bot, DIVG_bot_surf
+    INTEGER(C_INT32_T) :: nit
+    REAL(C_DOUBLE) :: rr, ms
+#endif
     TYPE(orctm_nh_vgrid_t) :: vg
     TYPE(orctm_nh_stats_t) :: st
     INTEGER(C_INT) :: ierr
@@ -174,6 +194,10 @@ CONTAINS
     ENDIF
     ierr = orctm_nh_divergence(nh_handle, UOO, UOO_1, VOE, VOE_1, WNO, WNO_1, ZO, DIVG)
     IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_divergence status ', ierr
+#ifdef CG2D4NONHY
+    ierr = orctm_nh_get_bottom(nh_handle, PNH_bot, DIVG_bot_surf, nit, rr, ms)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_get_bottom status ', ierr, ' rel.res ', rr
+#endif
     ierr = orctm_nh_solve_resident(nh_handle, st)
     IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_solve_resident status ', ierr, ' rel.res ', st%rel_res
     ierr = orctm_nh_project(nh_handle, PNH, UOO, VOE, WNO)

@@ -127,14 +127,14 @@ class NHSolver:
 
     def get_bottom(self) -> dict:
         """With bottom_solve: PNH_bot (IE,JE, halo included), DIVG_bot_surf (IE,JE, interior), and the
-        iterations and relative residual of the last 2-D bottom solve (orctm_nh_get_bottom)."""
+        iterations, relative residual and device time of the last 2-D bottom solve (orctm_nh_get_bottom)."""
         IE, JE, KE = self.shape
         p = np.zeros((IE, JE), order="F")
         d = np.zeros((IE, JE), order="F")
-        it, rr = ctypes.c_int32(), ctypes.c_double()
-        self._rc(self.lib.orctm_nh_get_bottom(self.h, _dp(p), _dp(d), ctypes.byref(it), ctypes.byref(rr)),
-                 "orctm_nh_get_bottom")
-        return {"pnh_bot": p, "divg_bot_surf": d, "iterations": it.value, "rel_res": rr.value}
+        it, rr, ms = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        self._rc(self.lib.orctm_nh_get_bottom(self.h, _dp(p), _dp(d), ctypes.byref(it), ctypes.byref(rr),
+                                              ctypes.byref(ms)), "orctm_nh_get_bottom")
+        return {"pnh_bot": p, "divg_bot_surf": d, "iterations": it.value, "rel_res": rr.value, "device_ms": ms.value}
 
     def project(self, pnh, uoo, voe, wno) -> None:
         """PNH halo, U/V pressure correction and WNO by continuity (ocvtot_nh.f90:151-230), in place."""
