@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
     ap.add_argument("--tile-rows", type=int, default=0, help="inner-kernel tile height: 0 = tuned, or 4, 8, 12")
     ap.add_argument("--sor-depth", type=int, default=0,
-                    help="troneu workload: half-sweeps per launch, 0 = tuned, or 4, 6, 8, 12")
+                    help="troneu workload: half-sweeps per launch, 0 = tuned, or 4, 6, 7, 8, 12")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (systems would shard "

@@ -261,7 +261,7 @@ typedef struct orctm_nh_sor_grid {
   int32_t cyclic_x, cyclic_y;  /* ICYCLI_X, ICYCLI_Y */
   int32_t iterations;          /* red-black sweeps per solve (250 in troneu.f90:103) */
   int32_t device;              /* HIP device ordinal */
-  int32_t half_sweeps;         /* half-sweeps per blocked device launch: 0 = the fastest of 4, 6, 8, 12, timed at
+  int32_t half_sweeps;         /* half-sweeps per blocked device launch: 0 = the fastest of 4, 6, 7, 8, 12, timed at
                                   creation; or one of them. Results are bit-identical for every choice */
   int32_t reserved;            /* 0 */
   const double* dlxp;          /* DLXP(IE,JE) */

@@ -22,8 +22,9 @@
 // the tile stores are contiguous rows. The LDS holds the region split by colour: plane (x + y) & 1,
 // row y >> 1, column x, padded by a row and a word on either side, so the four neighbours of a cell lie
 // in the other plane, consecutive lanes read consecutive words, and a lane's reads for all its pairs
-// are one base address plus constant offsets. The coefficients of a thread's cells stay in registers
-// for the whole launch, ordered by the colour that updates them.
+// are one base address plus constant offsets. B1O and SORPAR*FF of every cell sit beside them in LDS;
+// the four face coefficients of a thread's cells stay in registers for the whole launch, ordered by
+// the colour that updates them, which keeps a 512-thread block within 256 VGPRs at five pairs per thread.
 //
 // Regular and seam kernels. Unless a cyclic direction has an odd period, the mapped parity equals the
 // virtual one everywhere, so a lane's colour-C cells are all cell s0 ^ C of their pairs,
@@ -67,7 +68,7 @@ __host__ __device__ inline int tb_tile(int b, int nb) {
 }
 
 struct TbCo {
-  double b, w, ue, uw, vn, vs;  // B1O, SORPAR*FF, UF(I), UF(I-1), VF(J), VF(J-1)
+  double ue, uw, vn, vs;  // UF(I), UF(I-1), VF(J), VF(J-1)
 };
 
 template <int T, int RX, int RY, int NT, bool SEAM>
@@ -78,7 +79,8 @@ struct SorTb {
   static constexpr int DP = NT / RX;       // pair rows between consecutive pairs of a thread
   static constexpr int PAD = RX + 1;       // padding words before and after each plane
   static constexpr int PS = PL + 2 * PAD;  // plane stride
-  static constexpr int LDS = 2 * PS;       // doubles of LDS
+  static constexpr int BW = 2 * PS;        // start of the (B1O, SORPAR*FF) pairs, same plane order, no padding
+  static constexpr int LDS = 2 * PS + 4 * PL;  // doubles of LDS
   static constexpr int TXo = RX - 2 * T;   // output tile
   static constexpr int TYo = RY - 2 * T;
   static_assert(RY % 2 == 0 && NT % RX == 0 && PL % NT == 0 && NP >= 1 && NP <= 16,
@@ -96,6 +98,8 @@ struct SorTb {
   };
 
   __host__ __device__ static int at(int x, int y) { return PAD + ((x + y) & 1) * PS + (y >> 1) * RX + x; }
+  // index of the (B1O, SORPAR*FF) pair of cell (x, y)
+  __host__ __device__ static int bw(int x, int y) { return BW + 2 * (((x + y) & 1) * PL + (y >> 1) * RX + x); }
 
   static __host__ __device__ size_t p2(const TbGeom& g, int i, int j) {
     return (size_t)(i - 1) + (size_t)g.IE * (size_t)(j - 1);
@@ -143,7 +147,9 @@ struct SorTb {
         r.z[k][s] = in ? zv : 0.0;
         lz[at(x, y)] = r.z[k][s];
         // SORPAR*FF(I,J) is the first product of the source's expression
-        co[s] = upd ? TbCo{b, sorpar * f, ue, uw, vn, vs} : TbCo{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        lz[bw(x, y)] = upd ? b : 0.0;
+        lz[bw(x, y) + 1] = upd ? sorpar * f : 0.0;
+        co[s] = upd ? TbCo{ue, uw, vn, vs} : TbCo{0.0, 0.0, 0.0, 0.0};
         const uint32_t bit = upd ? 1u << (2 * k + s) : 0u;
         if (((mi + mj) & 1) == 0) r.m0 |= bit;
         else r.m1 |= bit;
@@ -159,10 +165,10 @@ struct SorTb {
     }
   }
 
-  __host__ __device__ static double cell(const TbCo& c, double zE, double zW, double zN, double zS, double z0,
-                                         double sorpai) {
-    const double s = c.b + c.ue * zE + c.uw * zW + c.vn * zN + c.vs * zS;
-    return c.w * s + sorpai * z0;
+  __host__ __device__ static double cell(const TbCo& c, double b, double w, double zE, double zW, double zN, double zS,
+                                         double z0, double sorpai) {
+    const double s = b + c.ue * zE + c.uw * zW + c.vn * zN + c.vs * zS;
+    return w * s + sorpai * z0;
   }
 
   // New values of this thread's cells of colour C, from the region as it stands (before the writes).
@@ -175,13 +181,15 @@ struct SorTb {
       // (in the pair row above or below) and the two horizontal ones come from the other plane.
       const int s = r.s0 ^ C, own = (x + s) & 1;
       const double* lo = lz + PAD + (1 - own) * PS + p0 * RX + x;  // the other plane at (x, pair row p0)
+      const double* lb = lz + BW + 2 * (own * PL + p0 * RX + x);   // this cell's (B1O, SORPAR*FF)
       const int dv = s ? RX : -RX;  // the pair row of the vertical neighbour outside the pair
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const int o = k * DP * RX;
         const double zE = lo[o + 1], zW = lo[o - 1], zV = lo[o + dv];
+        const double b = lb[2 * o], w = lb[2 * o + 1];
         const double z0 = s ? r.z[k][1] : r.z[k][0], zP = s ? r.z[k][0] : r.z[k][1];
-        const double v = cell(r.c[C][k], zE, zW, s ? zV : zP, s ? zP : zV, z0, sorpai);
+        const double v = cell(r.c[C][k], b, w, zE, zW, s ? zV : zP, s ? zP : zV, z0, sorpai);
         r.n0[k] = v;
         // only this thread reads its registers, and the partner is not updated in this colour
         if ((C ? r.m1 : r.m0) >> (2 * k + s) & 1u) {
@@ -202,8 +210,8 @@ struct SorTb {
           const int y = 2 * p + s;
           // coefficient slot of cell s: C' with s0 ^ C' == s
           const TbCo& c = (r.s0 ^ s) ? r.c[1][k] : r.c[0][k];
-          const double v = cell(c, lz[at(x + 1, y)], lz[at(x - 1, y)], lz[at(x, y + 1)], lz[at(x, y - 1)],
-                                lz[at(x, y)], sorpai);
+          const double v = cell(c, lz[bw(x, y)], lz[bw(x, y) + 1], lz[at(x + 1, y)], lz[at(x - 1, y)],
+                                lz[at(x, y + 1)], lz[at(x, y - 1)], lz[at(x, y)], sorpai);
           if (s == 0) r.n0[k] = v;
           else r.n1[k] = v;
         }
@@ -249,11 +257,12 @@ struct SorTb {
   }
 };
 
-// The half-sweeps per launch that are compiled, with their regions: T, RX, RY, threads. Four pairs per
-// thread keep the coefficients in about 200 VGPRs, so blocks stay at 512 threads or fewer.
+// The half-sweeps per launch that are compiled, with their regions: T, RX, RY, threads (pairs per
+// thread = RX*RY/2/threads).
 #define ORCTM_SOR_TB_VARIANTS(X) \
   X(4, 64, 32, 256)              \
-  X(6, 64, 48, 384)              \
+  X(6, 64, 80, 512)              \
+  X(7, 64, 80, 512)              \
   X(8, 64, 64, 512)              \
   X(12, 64, 64, 512)
 

@@ -72,7 +72,7 @@ def _troneu_case(g, iterations=None, half_sweeps=0):
     return b, info, bits_equal(z1o, z_ref), bits_equal(b1o, b1_ref)
 
 
-@pytest.mark.parametrize("half_sweeps", [4, 6, 8, 12])
+@pytest.mark.parametrize("half_sweeps", [4, 6, 7, 8, 12])
 @pytest.mark.parametrize("name", ["regional", "cyclic_xy", "transect"])
 def test_troneu_every_launch_depth(name, half_sweeps):
     print(f"[sor depth] {name} {half_sweeps}", flush=True)
@@ -83,13 +83,13 @@ def test_troneu_every_launch_depth(name, half_sweeps):
     assert z_ok and b_ok
 
 
-@pytest.mark.parametrize("half_sweeps", [0, 4, 6, 8, 12])
+@pytest.mark.parametrize("half_sweeps", [0, 4, 6, 7, 8, 12])
 @pytest.mark.parametrize("name", list(ODD))
 def test_troneu_odd_cyclic_periods(name, half_sweeps):
     print(f"[sor odd] {name} {half_sweeps}", flush=True)
     g, _ = synth.build(ODD[name])
     b, info, z_ok, b_ok = _troneu_case(g, half_sweeps=half_sweeps)
-    assert info["half_sweeps"] in (4, 6, 8, 12)
+    assert info["half_sweeps"] in (4, 6, 7, 8, 12)
     assert z_ok and b_ok
 
 
