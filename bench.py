@@ -31,15 +31,23 @@ from orctm_amd.nh import NHSolver, options  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes of one launch, per cell and per (i, j) column, for word size W (4 FP32, 8 FP64)
-# and line-preconditioner factor size F (2: fp16 in FP32 mode, 8 in FP64 mode).
-# k_bwd (back substitution + p update + 7-point matvec + p.q + the previous iteration's x += alpha p):
-#   reads w, p_old, x, ZW, ZP (5 words), l (F) and the uint16 coupling flags per cell, writes p, q, x
-#   (3 words); per column it reads the 7 metric products and the bottom-face ZW (8 words).
+# The roofline figure of the hot kernel is the survey's matvec y = A x: 7 coefficients + x + y per cell of
+# the box, 36 B in FP32 (72 B in FP64). k_bwd performs that matvec (and the back substitution, the
+# search-direction update, the deferred x update and p.q) per launch.
+MATVEC_BYTES = {4: 36, 8: 72}
+
+# The bytes each kernel itself streams per launch, per cell and per (i, j) column, for word size W
+# (4 FP32, 8 FP64) and line-preconditioner factor size F (2: fp16 in FP32 mode, 8 in FP64 mode).
+# k_bwd reads w, p_old, x (3 words), l (F) and the uint16 coupling flags per cell and writes p, q, x
+#   (3 words); per column the 7 metric products, and either the ZW/ZP window (6 words + an int) when
+#   the system uses the per-level table (stats zw_table = 1) or, per cell, ZW and ZP (2 words) and the
+#   bottom-face ZW per column otherwise.
 # k_fwd (r update + forward elimination + r.z, r.r): reads r, q (2 words), l, invD (2 F), writes r, w;
 #   in FP32 mode also the per-column scale of invD (4 B per column).
-def bwd_bytes(cells, columns, W):
+def bwd_bytes(cells, columns, W, table=True):
     F = 2 if W == 4 else 8
+    if table:
+        return cells * (6 * W + F + 2) + columns * (13 * W + 4)
     return cells * (8 * W + F + 2) + columns * 8 * W
 
 
@@ -179,7 +187,9 @@ def main():
     up_avg_ms = prof["update_ms"] / max(prof["update_launches"], 1)
     # the first iteration of each inner solve (one per refinement step) has no pending x update
     first_frac = mx["stats"][-1]["outer_iters"] / max(mx["stats"][-1]["inner_iters"], 1)
-    mv_bytes = int(bwd_bytes(cells, columns, 4) - first_frac * 8 * cells)
+    table = mx["stats"][-1].get("zw_table", 0) == 1
+    kern_bytes = int(bwd_bytes(cells, columns, 4, table) - first_frac * 8 * cells)
+    mv_bytes = MATVEC_BYTES[4] * cells
     achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
     prof64 = f64["prof"]
     mv64_ms = prof64["matvec_ms"] / max(prof64["matvec_launches"], 1)
@@ -217,16 +227,21 @@ def main():
         "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "latency_ms_p50_fp64": round(pct(f64["lat"], 50), 3),
         "latency_ms_p99_fp64": round(pct(f64["lat"], 99), 3),
-        "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (back substitution + p update + 7-point matvec + p.q)",
+        "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (7-point matvec + back substitution + p update + p.q)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
-                     "bytes_per_launch": mv_bytes, "bytes_per_cell": 36, "bytes_per_column": 32,
-                     "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
+                     "bytes_per_launch": mv_bytes, "bytes_per_cell": MATVEC_BYTES[4],
+                     "bytes_note": "the survey's matvec: 7 coefficients + x + y per cell (FP32)",
+                     "kernel_bytes_per_launch": kern_bytes, "kernel_GBps": round(kern_bytes / (mv_avg_ms * 1e-3) / 1e9, 1),
+                     "zw_table": int(table), "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
         "other_kernels": {
             "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 20, "bytes_per_column": 4,
                              "GBps": round(fwd_bytes(cells, columns, 4) / (up_avg_ms * 1e-3) / 1e9, 1)},
-            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "bytes_per_cell": 74, "bytes_per_column": 64,
-                              "GBps": round(bwd_bytes(cells, columns, 8) / (mv64_ms * 1e-3) / 1e9, 1)},
+            "k_bwd<double>": {"avg_launch_ms": round(mv64_ms, 5), "matvec_bytes_per_cell": MATVEC_BYTES[8],
+                              "matvec_GBps": round(MATVEC_BYTES[8] * cells / (mv64_ms * 1e-3) / 1e9, 1),
+                              "kernel_GBps": round(bwd_bytes(cells, columns, 8, f64["stats"][-1].get("zw_table", 0) == 1)
+                                                   / (mv64_ms * 1e-3) / 1e9, 1)},
             "k_fwd<double>": {"avg_launch_ms": round(up64_ms, 5), "bytes_per_cell": 48,
                               "GBps": round(fwd_bytes(cells, columns, 8) / (up64_ms * 1e-3) / 1e9, 1)},
         },
@@ -291,7 +306,7 @@ def bench_batch(args):
         inner = sum(x["inner_iters"] for x in st)
         # cell-iterations actually processed per solve (converged systems drop out of later launches)
         cell_iters = sum(x["inner_iters"] * g.cells for x, (g, _) in zip(st, systems))
-        mv_bytes = sum(x["inner_iters"] * bwd_bytes(g.cells, g.nx * g.ny, 4) for x, (g, _) in zip(st, systems))
+        mv_bytes = sum(x["inner_iters"] * MATVEC_BYTES[4] * g.cells for x, (g, _) in zip(st, systems))
         # one-by-one reference timing of the same systems (sequential, same kernels)
         t1 = time.perf_counter()
         for s in solvers:
@@ -331,8 +346,8 @@ def bench_batch(args):
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "36 B per active cell-iteration + 32 B per active column-iteration over the summed "
-                             "matvec launch time"},
+                     "note": "the survey's matvec bytes (36 B per cell, FP32) per active cell-iteration over the "
+                             "summed matvec launch time"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

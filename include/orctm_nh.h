@@ -112,7 +112,11 @@ typedef struct orctm_nh_stats {
   double bnorm;         /* ||b||_2 */
   double solve_ms;      /* device time of the solve (HIP events), excluding host<->device copies */
   double total_ms;      /* host wall time of the call, including copies */
-  double reserved[4];
+  int32_t zw_table;     /* 1: the iteration kernels read ZW/ZP from the wet column's per-level table and
+                           each column's 3-level window around its bottom (checked bit for bit at create);
+                           0: from the per-cell planes */
+  int32_t reserved_i;
+  double reserved[3];
 } orctm_nh_stats_t;
 
 typedef struct orctm_nh orctm_nh_t;

@@ -51,14 +51,15 @@ class StatsC(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int32), ("outer_iters", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
         ("tile_rows", ctypes.c_int32), ("rel_res", ctypes.c_double), ("bnorm", ctypes.c_double),
-        ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("reserved", ctypes.c_double * 4),
+        ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("zw_table", ctypes.c_int32),
+        ("reserved_i", ctypes.c_int32), ("reserved", ctypes.c_double * 3),
     ]
 
     def as_dict(self):
         return {"status": STATUS_NAMES[self.status] if 0 <= self.status < len(STATUS_NAMES) else self.status,
                 "outer_iters": self.outer_iters, "inner_iters": self.inner_iters, "rel_res": self.rel_res,
                 "bnorm": self.bnorm, "solve_ms": self.solve_ms, "total_ms": self.total_ms,
-                "tile_rows": self.tile_rows}
+                "tile_rows": self.tile_rows, "zw_table": self.zw_table}
 
 
 class ServerOptionsC(ctypes.Structure):

@@ -685,11 +685,12 @@ int outer_status(const Outer& o) {
   }
 }
 
-void fill_stats(const Outer& o, int tile_rows, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
+void fill_stats(const Outer& o, int tile_rows, int zw_table, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
   if (!st) return;
   memset(st, 0, sizeof(*st));
   st->status = outer_status(o);
   st->tile_rows = tile_rows;
+  st->zw_table = zw_table;
   st->outer_iters = o.outer;
   st->inner_iters = o.inner_total;
   st->rel_res = o.rel;
@@ -800,15 +801,29 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   ALLOC(zp, NP * sizeof(double), const double*);
   ALLOC(flg, NP * sizeof(uint16_t), const uint16_t*);
   ALLOC(col, 7 * (size_t)h->plane * sizeof(double), const double*);
+  ALLOC(tw, (size_t)(h->nz + 1) * sizeof(double), const double*);
+  ALLOC(tp, (size_t)h->nz * sizeof(double), const double*);
+  ALLOC(xk, (size_t)h->plane * sizeof(int), const int*);
+  ALLOC(xw, 3 * (size_t)h->plane * sizeof(double), const double*);
+  ALLOC(xp, 3 * (size_t)h->plane * sizeof(double), const double*);
   if (h->fp32) {
     ALLOC(zw_in, NP1 * sizeof(float), const void*);
     ALLOC(zp_in, NP * sizeof(float), const void*);
     ALLOC(col_in, 7 * (size_t)h->plane * sizeof(float), const void*);
+    ALLOC(tw_in, (size_t)(h->nz + 1) * sizeof(float), const void*);
+    ALLOC(tp_in, (size_t)h->nz * sizeof(float), const void*);
+    ALLOC(xw_in, 3 * (size_t)h->plane * sizeof(float), const void*);
+    ALLOC(xp_in, 3 * (size_t)h->plane * sizeof(float), const void*);
   } else {
     s.zw_in = s.zw;
     s.zp_in = s.zp;
     s.col_in = s.col;
+    s.tw_in = s.tw;
+    s.tp_in = s.tp;
+    s.xw_in = s.xw;
+    s.xp_in = s.xp;
   }
+  s.tab = 0;
   // line-preconditioner factors: fp16 bits + a per-column scale for FP32, FP64 values otherwise
   const size_t fs = h->fp32 ? sizeof(uint16_t) : sizeof(double);
   ALLOC(invD, NP * fs, const void*);
@@ -832,7 +847,8 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   Sys* d_sys1 = nullptr;
   Tile* d_tiles1 = nullptr;
   int* d_bad = nullptr;
-  int bad = 0, rc = ORCTM_NH_OK;
+  int* d_zbad = nullptr;
+  int bad = 0, zbad = 1, rc = ORCTM_NH_OK;
   std::vector<Tile> tl = system_tiles(0, h->nx, h->ny, TY);
   e = hipMalloc(&d_dlxp, n2 * 8);
   if (e == hipSuccess) e = hipMalloc(&d_dlyp, n2 * 8);
@@ -844,6 +860,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   if (e == hipSuccess) e = hipMalloc(&d_sys1, sizeof(Sys));
   if (e == hipSuccess) e = hipMalloc(&d_tiles1, sizeof(Tile) * tl.size());
   if (e == hipSuccess) e = hipMalloc(&d_bad, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&d_zbad, sizeof(int));
   if (e == hipSuccess) e = upload(d_dlxp, g->dlxp, n2 * 8);
   if (e == hipSuccess) e = upload(d_dlyp, g->dlyp, n2 * 8);
   if (e == hipSuccess) e = upload(d_dlxu, g->dlxu, nu * 8);
@@ -854,6 +871,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   if (e == hipSuccess) e = upload(d_sys1, &s, sizeof(Sys));
   if (e == hipSuccess) e = upload(d_tiles1, tl.data(), sizeof(Tile) * tl.size());
   if (e == hipSuccess) e = hipMemset(d_bad, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(d_zbad, 0, sizeof(int));
   if (e == hipSuccess) {
     FortranIn f{};
     f.IE = g->ie;
@@ -876,8 +894,10 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
     f.dzw = d_dzw;
     e = launch_assemble(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_bad, h->d_smax);
     if (e == hipSuccess) e = launch_line_factor(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size());
+    if (e == hipSuccess) e = launch_zwindow(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_zbad);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&zbad, d_zbad, sizeof(int), hipMemcpyDeviceToHost);
   }
   if (e != hipSuccess) rc = map_err(e);
   else if (bad) rc = ORCTM_NH_ERR_INVALID;
@@ -891,7 +911,9 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   hipFree(d_sys1);
   hipFree(d_tiles1);
   hipFree(d_bad);
+  hipFree(d_zbad);
   if (rc != ORCTM_NH_OK) return rc;
+  s.tab = zbad ? 0 : 1;
   e = ctx_build(h->ctx, {s}, {h->opts.tile_rows}, h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device,
                 h->opts.profile != 0);
   return map_err(e);
@@ -1070,7 +1092,7 @@ int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
   float ms = 0.f;
   hipError_t e = ctx_solve(h->ctx, &ms);
   if (e != hipSuccess) return map_err(e);
-  fill_stats(h->ctx.h_outer[0], h->ctx.heights[0], st, ms, now_ms() - t0);
+  fill_stats(h->ctx.h_outer[0], h->ctx.heights[0], h->sys.tab, st, ms, now_ms() - t0);
   return outer_status(h->ctx.h_outer[0]);
 }
 
@@ -1532,7 +1554,7 @@ int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
   for (int q = 0; q < b->ctx.nsys; ++q) {
     const int s = outer_status(b->ctx.h_outer[q]);
     if (rc == ORCTM_NH_OK && s != ORCTM_NH_OK) rc = s;
-    if (st) fill_stats(b->ctx.h_outer[q], b->ctx.heights[q], st + q, ms, tot);
+    if (st) fill_stats(b->ctx.h_outer[q], b->ctx.heights[q], b->members[q]->sys.tab, st + q, ms, tot);
   }
   return rc;
 }

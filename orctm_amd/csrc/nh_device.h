@@ -63,6 +63,20 @@ struct Sys {
   const void* zw_in;    // the same in the inner precision (FP32 copies for MIXED; aliases for FP64)
   const void* zp_in;
   const void* col_in;
+  // ZW/ZP as a per-level table plus a 3-level window per column: on a z-level grid they equal the
+  // values of a column wet down to KE with full cells everywhere except around the column's bottom
+  // cell. `tab` = 1 when the set-up found every column of the system so (bit for bit, both precisions);
+  // the iteration kernels then read the table and the window instead of the per-cell planes.
+  int tab;
+  const double* tw;     // nz+1 ZW of the wet column
+  const double* tp;     // nz ZP of the wet column
+  const void* tw_in;    // the same in the inner precision
+  const void* tp_in;
+  const int* xk;        // per column: first level of the window
+  const double* xw;     // 3 planes: ZW at levels xk, xk+1, xk+2
+  const double* xp;     // 3 planes: ZP at levels xk, xk+1, xk+2
+  const void* xw_in;    // the same in the inner precision
+  const void* xp_in;
   const void* invD;     // line preconditioner 1/D_k: FP64, or fp16 bits scaled by dscale (FP32)
   const void* L;        // line preconditioner l_k: FP64, or fp16 bits (FP32)
   const float* dscale;  // FP32 only: per-column power of two that multiplies the stored invD

@@ -19,6 +19,10 @@ struct FortranIn {
 hipError_t launch_assemble(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles,
                            const FortranIn& f, int* d_bad, unsigned long long* d_smax_bits);
 hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+// The per-level ZW/ZP table of the wet column and each column's 3-level window (Sys::tab); *d_bad set
+// when some column's deviations from the table do not fit in 3 consecutive levels.
+hipError_t launch_zwindow(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                          const FortranIn& f, int* d_bad);
 // b = DIVG*WETO from a device copy of DIVG(IE,JE,KE); ||b|| into Outer.nb.
 hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
                       const double* d_divg, int IE, int JE);

@@ -203,6 +203,118 @@ __device__ __forceinline__ void row_coef(const T* cc, T zwk, T zwk1, T zpk, unsi
   c7[6] = (f & (1u << 6)) ? zwk1 * zpk : T(0);
 }
 
+// ZW/ZP of one column from the per-level table and the column's 3-level window (Sys::tab).
+template <typename T>
+struct ZCol {
+  gptr<const T> tw, tp;
+  int kx;
+  T w0, w1, w2, p0, p1, p2;
+  __device__ void load(gptr<const T> tw_, gptr<const T> tp_, const int* xk, gptr<const T> xw, gptr<const T> xp,
+                       size_t P, size_t off) {
+    tw = tw_;
+    tp = tp_;
+    kx = xk[off];
+    w0 = xw[off];
+    w1 = xw[P + off];
+    w2 = xw[2 * P + off];
+    p0 = xp[off];
+    p1 = xp[P + off];
+    p2 = xp[2 * P + off];
+  }
+  __device__ T zw(int m) const {
+    const int d = m - kx;
+    return d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : tw[m];
+  }
+  __device__ T zp(int m) const {
+    const int d = m - kx;
+    return d == 0 ? p0 : d == 1 ? p1 : d == 2 ? p2 : tp[m];
+  }
+};
+
+// The wet column's ZW/ZP (mo_petsc.f90:557-590 with DDPO = DZW on every level), with the assembly's
+// expressions, so that full columns reproduce the assembled values bit for bit. One thread.
+template <typename T>
+__global__ void k_ztab(const Sys* __restrict__ systems, FortranIn f) {
+  const Sys& s = systems[0];
+  if (threadIdx.x != 0) return;
+  gptr<double> tw = G<double>(s.tw);
+  gptr<double> tp = G<double>(s.tp);
+  gptr<T> twi = G<T>(s.tw_in);
+  gptr<T> tpi = G<T>(s.tp_in);
+  const bool copy_in = (const void*)s.tw_in != (const void*)s.tw;
+  const int KE = f.KE;
+  double zwK = 1.0 / (f.dzw[0] / 2.0);
+  tw[0] = zwK;
+  if (copy_in) twi[0] = (T)zwK;
+  for (int K = 1; K <= KE; ++K) {
+    const double dd = K < KE ? (f.dzw[K - 1] + f.dzw[K]) / 2.0 : f.dzw[KE - 1];
+    const double zwK1 = 1.0 / dd;
+    const double zpK = 1.0 / ((1.0 / zwK + 1.0 / zwK1) / 2.0);
+    tw[K] = zwK1;
+    tp[K - 1] = zpK;
+    if (copy_in) {
+      twi[K] = (T)zwK1;
+      tpi[K - 1] = (T)zpK;
+    }
+    zwK = zwK1;
+  }
+}
+
+// Per column: the levels where the assembled ZW/ZP (either precision) differ from the table, which
+// must fit in a window of 3 consecutive levels; the window's values. *bad set if some column does not fit.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zwindow(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                                int* bad) {
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
+  if (i >= s.nx || j >= s.ny) return;
+  const int nz = s.nz;
+  const size_t P = s.plane, col = (size_t)j * s.pitch + i;
+  gptr<const double> zw = G<const double>(s.zw), zp = G<const double>(s.zp), tw = G<const double>(s.tw),
+                     tp = G<const double>(s.tp);
+  gptr<const T> zwi = G<const T>(s.zw_in), zpi = G<const T>(s.zp_in), twi = G<const T>(s.tw_in),
+                tpi = G<const T>(s.tp_in);
+  auto same64 = [](double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); };
+  auto sameT = [](T a, T b) {
+    if (sizeof(T) == 8) return __double_as_longlong((double)a) == __double_as_longlong((double)b);
+    return __float_as_uint((float)a) == __float_as_uint((float)b);
+  };
+  int lo = nz + 1, hi = -1;
+  for (int m = 0; m <= nz; ++m) {
+    const size_t o = (size_t)m * P + col;
+    bool d = !same64(zw[o], tw[m]) || !sameT(zwi[o], twi[m]);
+    if (m < nz) d = d || !same64(zp[o], tp[m]) || !sameT(zpi[o], tpi[m]);
+    if (d) {
+      lo = lo < m ? lo : m;
+      hi = m;
+    }
+  }
+  int kx = 0;
+  if (hi >= 0) {
+    kx = lo;
+    const int top = nz >= 2 ? nz - 2 : 0;  // keep the window within ZW's nz+1 levels
+    if (kx > top) kx = top;
+    if (hi > kx + 2) atomicOr(bad, 1);
+  }
+  int* xk = const_cast<int*>(s.xk);
+  gptr<double> xw = G<double>(s.xw), xp = G<double>(s.xp);
+  gptr<T> xwi = G<T>(s.xw_in), xpi = G<T>(s.xp_in);
+  const bool copy_in = (const void*)s.xw_in != (const void*)s.xw;
+  xk[col] = kx;
+  for (int c = 0; c < 3; ++c) {
+    const int m = kx + c;
+    const double w = m <= nz ? zw[(size_t)m * P + col] : 0.0;
+    const double p = m < nz ? zp[(size_t)m * P + col] : 0.0;
+    xw[c * P + col] = w;
+    xp[c * P + col] = p;
+    if (copy_in) {
+      xwi[c * P + col] = m <= nz ? zwi[(size_t)m * P + col] : T(0);
+      xpi[c * P + col] = m < nz ? zpi[(size_t)m * P + col] : T(0);
+    }
+  }
+}
+
 // Assembly into the compact form. ZW/ZP follow mo_petsc.f90:557-590, the per-column products
 // mo_petsc.f90:593-599,610-615, the flag bits the branches of mo_petsc.f90:623-765.
 template <typename T>
@@ -576,6 +688,9 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
       gptr<const double> zw = G<const double>(s.zw);
       gptr<const double> zp = G<const double>(s.zp);
       gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+      const bool tab = s.tab != 0;
+      ZCol<double> zc;
+      if (tab) zc.load(G<const double>(s.tw), G<const double>(s.tp), s.xk, G<const double>(s.xw), G<const double>(s.xp), P, col);
       double cc[7];
       load_col(s, col, cc);
       // horizontal neighbour columns (periodic wrap where cyclic; absent ones read this column and
@@ -596,8 +711,8 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         const int kk = k < nz ? k : nz - 1;
         const size_t o = (size_t)kk * P;
         v.b = ld_nt<double>(b, col + o);
-        v.zp = ld_nt<double>(zp, col + o);
-        v.zw1 = ld_nt<double>(zw, col + o + P);
+        v.zp = tab ? zc.zp(kk) : ld_nt<double>(zp, col + o);
+        v.zw1 = tab ? zc.zw(kk + 1) : ld_nt<double>(zw, col + o + P);
         v.f = ld_nt<uint16_t>(flg, col + o);
         v.pd = p[col + (kk < nz - 1 ? o + P : o)];
         v.pw = p[cw + o];
@@ -605,7 +720,7 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         v.pn = p[cn + o];
         v.ps = p[cs + o];
       };
-      double xu = 0.0, x0 = p[col], zwk = ld_nt<double>(zw, col);
+      double xu = 0.0, x0 = p[col], zwk = tab ? zc.zw(0) : ld_nt<double>(zw, col);
       auto compute = [&](int k, const Lv& v) {
         const double xd = k < nz - 1 ? v.pd : 0.0;
         double c7[7];
@@ -842,10 +957,14 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     unsigned f;
   };
   T cc[7];
+  const bool tab = s.tab != 0;
+  ZCol<T> zcol;
   if (centre) {
     gptr<const T> cpi = G<const T>(s.col_in);
 #pragma unroll
     for (int c = 0; c < 7; ++c) cc[c] = cpi[c * P + off];
+    if (tab)
+      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), P, off);
   }
   T zw_below = T(0);  // ZW at the bottom face of the current level (top face of the level below)
   (void)NP;
@@ -859,9 +978,15 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   };
   auto load_c = [&](int m, Stage& st) {
     if (!centre) return;
-    const size_t cm = (size_t)(m > 0 ? m : 0) * P + off;
-    st.zw = ld_nt<T>(zwi, cm);
-    st.zp = ld_nt<T>(zpi, cm);
+    const int mm = m > 0 ? m : 0;
+    const size_t cm = (size_t)mm * P + off;
+    if (tab) {
+      st.zw = zcol.zw(mm);
+      st.zp = zcol.zp(mm);
+    } else {
+      st.zw = ld_nt<T>(zwi, cm);
+      st.zp = ld_nt<T>(zpi, cm);
+    }
     st.f = ld_nt<uint16_t>(flg, cm);
   };
   // back substitution and search-direction update of level m for this thread's column
@@ -905,7 +1030,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     p0 = pm1;
   };
   Stage s0, s1;
-  if (centre) zw_below = zwi[(size_t)nz * P + off];  // ZW(KE+1), the bottom face of the last level
+  if (centre) zw_below = tab ? zcol.zw(nz) : zwi[(size_t)nz * P + off];  // ZW(KE+1), the last level's bottom face
   load_a(nz - 1, s1);
   load_a(nz - 2, s0);
   load_c(nz - 1, s0);
@@ -1049,6 +1174,17 @@ hipError_t launch_assemble(hipStream_t st, bool fp32, const Sys* d_sys, const Ti
                            const FortranIn& f, int* d_bad, unsigned long long* d_smax_bits) {
   if (fp32) LAUNCH(k_assemble<float>, ntiles, st, d_sys, d_tiles, f, d_bad, d_smax_bits);
   LAUNCH(k_assemble<double>, ntiles, st, d_sys, d_tiles, f, d_bad, d_smax_bits);
+}
+
+hipError_t launch_zwindow(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                          const FortranIn& f, int* d_bad) {
+  if (ntiles <= 0) return hipSuccess;
+  if (fp32) hipLaunchKernelGGL(k_ztab<float>, dim3(1), dim3(64), 0, st, d_sys, f);
+  else hipLaunchKernelGGL(k_ztab<double>, dim3(1), dim3(64), 0, st, d_sys, f);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (fp32) LAUNCH(k_zwindow<float>, ntiles, st, d_sys, d_tiles, d_bad);
+  LAUNCH(k_zwindow<double>, ntiles, st, d_sys, d_tiles, d_bad);
 }
 
 hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles) {

@@ -39,7 +39,8 @@ MODULE orctm_nh_c
   TYPE, BIND(C) :: orctm_nh_stats_t
     INTEGER(C_INT32_T) :: status, outer_iters, inner_iters, tile_rows
     REAL(C_DOUBLE) :: rel_res, bnorm, solve_ms, total_ms
-    REAL(C_DOUBLE) :: reserved(4)
+    INTEGER(C_INT32_T) :: zw_table, reserved_i
+    REAL(C_DOUBLE) :: reserved(3)
   END TYPE
 
   TYPE, BIND(C) :: orctm_nh_sor_grid_t

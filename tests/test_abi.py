@@ -57,7 +57,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         pytest.skip("no gcc")
     checks = {"orctm_nh_grid_t": (abi.GridC, ["ie", "dlxp", "dzw", "cyclic_x", "open_s"]),
               "orctm_nh_options_t": (abi.OptionsC, ["precision", "rtol", "check_every", "profile", "tile_rows"]),
-              "orctm_nh_stats_t": (abi.StatsC, ["status", "tile_rows", "rel_res", "total_ms"]),
+              "orctm_nh_stats_t": (abi.StatsC, ["status", "tile_rows", "rel_res", "total_ms", "zw_table"]),
               "orctm_nh_profile_t": (abi.ProfileC, ["matvec_ms", "cells"]),
               "orctm_nh_server_options_t": (abi.ServerOptionsC, ["hbm_budget", "max_active", "device"]),
               "orctm_nh_request_stats_t": (abi.RequestStatsC, ["status", "rel_res", "t_done_ms"]),
