@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
     ap.add_argument("--tile-rows", type=int, default=0, help="inner-kernel tile height: 0 = tuned, or 4, 8, 12")
+    ap.add_argument("--zw-table", type=int, default=0, choices=[-1, 0, 1],
+                    help="where the iteration kernels read ZW/ZP: 0 = by size, 1 = table, -1 = per-cell planes")
     ap.add_argument("--sor-depth", type=int, default=0,
                     help="troneu workload: half-sweeps per launch, 0 = tuned, or 4, 6, 7, 8, 12")
     args = ap.parse_args()
@@ -160,7 +162,7 @@ def main():
     sols = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         # timed solves: the production path (inner iterations replayed from the captured hipGraph)
-        s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows))
+        s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table))
         s.set_rhs(divg)
         log(f"[bench] {name}: device bytes {s.device_bytes() / 1e9:.2f} GB")
         wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
@@ -275,7 +277,7 @@ def bench_batch(args):
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         solvers = []
         for g, divg in systems:
-            s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows))
+            s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table))
             s.set_rhs(divg)
             solvers.append(s)
         batch = NHBatch(solvers)
@@ -372,7 +374,7 @@ def bench_transect(args):
     cpu_s = (time.perf_counter() - t) / reps
     res = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        s = NHSolver(g, options(precision=prec, warm_start=False))
+        s = NHSolver(g, options(precision=prec, warm_start=False, zw_table=args.zw_table))
         s.set_rhs(divg)
         wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
@@ -391,6 +393,7 @@ def bench_transect(args):
         "fp64_solves_per_s": round(args.steps / res["fp64"]["wall"], 3),
         "inner_iters_mixed": mx["stats"][-1]["inner_iters"], "inner_iters_fp64": res["fp64"]["stats"][-1]["inner_iters"],
         "oracle_rel_res_mixed": mx["rel"], "oracle_rel_res_fp64": res["fp64"]["rel"],
+        "zw_table": mx["stats"][-1]["zw_table"],
         "rel_err_vs_cpu_fgmres": {k: v["err"] for k, v in res.items()},
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "cpu_baseline": {"value": round(1.0 / cpu_s, 4), "unit": "solves/s", "cores": oracle.num_threads(),
@@ -416,7 +419,7 @@ def bench_serve(args):
         if args.scale != 1.0:
             nx, ny = max(8, int(nx * args.scale)), max(8, int(ny * args.scale))
         g, divg = synth.build(synth.GridSpec(nx, ny, nz, seed=synth.BASE_SEED + 500 + q))
-        m = NHSolver(g, options(precision=abi.MIXED, warm_start=True))
+        m = NHSolver(g, options(precision=abi.MIXED, warm_start=True, zw_table=args.zw_table))
         m.set_rhs(divg)
         for a in ("dlxp", "dlyp", "dlxu", "dlyv", "ddpo", "weto"):
             setattr(g, a, None)

@@ -99,7 +99,13 @@ typedef struct orctm_nh_options {
                            and the corrections of DIVG by PNH_bot before the right-hand side is formed
                            (ocvtot_nh.f90:64-147, mo_petsc.f90:147-208, 252-374, 401-538). Default 0, as
                            the shipped Makefile builds (-DZZCG2D4NONHY). */
-  int32_t reserved[3];
+  int32_t zw_table;     /* where the iteration kernels read ZW/ZP (stats.zw_table reports the choice):
+                           0 (default) = from the per-level table and per-column window when they
+                           reproduce every column (checked at create) and the solve launches cover at
+                           least 2^20 cells (one system, or a batch in total), else the per-cell planes;
+                           1 = the table whenever it reproduces every column; -1 = always the planes.
+                           Results are the same bits for every choice. */
+  int32_t reserved[2];
 } orctm_nh_options_t;
 
 /* Per-solve statistics. */

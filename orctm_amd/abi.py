@@ -35,7 +35,8 @@ class OptionsC(ctypes.Structure):
         ("precision", ctypes.c_int32), ("maxit", ctypes.c_int32), ("max_outer", ctypes.c_int32),
         ("warm_start", ctypes.c_int32), ("rtol", ctypes.c_double), ("rtol_inner", ctypes.c_double),
         ("check_every", ctypes.c_int32), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("tile_rows", ctypes.c_int32), ("bottom_solve", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("tile_rows", ctypes.c_int32), ("bottom_solve", ctypes.c_int32), ("zw_table", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

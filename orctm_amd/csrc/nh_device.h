@@ -65,8 +65,9 @@ struct Sys {
   const void* col_in;
   // ZW/ZP as a per-level table plus a 3-level window per column: on a z-level grid they equal the
   // values of a column wet down to KE with full cells everywhere except around the column's bottom
-  // cell. `tab` = 1 when the set-up found every column of the system so (bit for bit, both precisions);
-  // the iteration kernels then read the table and the window instead of the per-cell planes.
+  // cell. The set-up sets `tab` = 1 when it found every column of the system so (bit for bit, both
+  // precisions); a solve context keeps it or clears it (options.zw_table), and with it set the
+  // iteration kernels read the table and the window instead of the per-cell planes.
   int tab;
   const double* tw;     // nz+1 ZW of the wet column
   const double* tp;     // nz ZP of the wet column

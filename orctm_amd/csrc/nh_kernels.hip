@@ -883,8 +883,17 @@ constexpr int bwd_threads() {
   return 64 * (H + 3);
 }
 
+// Levels the matvec kernel's loads run ahead of their use (the depth of its software pipeline).
+// Launches of many tiles hide the latency of a load behind other blocks' work, and one level ahead
+// keeps registers low; the low tiles (4 and 8 rows) serve systems with few tiles, where each block's
+// walk down its columns is a chain of dependent levels and only deeper prefetch overlaps them.
+template <int H>
+constexpr int bwd_depth() {
+  return H == 12 ? 1 : 4;
+}
+
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
-template <typename T, int H>
+template <typename T, int H, int D>
 __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                 int raw) {
   const Tile t = tiles[blockIdx.x];
@@ -947,10 +956,11 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   const bool val = slot && map_ij(mi, mj, nx, ny, s.cyc_x, s.cyc_y);
   const size_t off = val ? (size_t)mj * IP + mi : 0;
 
-  // Software pipeline over levels (bottom-up) in two register sets used alternately: while level k
-  // is finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its row
-  // data), set `nxt` receives w/l/p_old of level k-2 and the row data (ZW, ZP, flags) of level k-1.
-  // The 7 inner coefficients are rebuilt from the row data and this column's metric products.
+  // Software pipeline over levels (bottom-up) in D+1 register sets used in turn: while level k is
+  // finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its row data), set
+  // `nxt` receives w/l/p_old of level k-1-D and the row data (ZW, ZP, flags) of level k-D, the set
+  // that step k-D will finish with. The 7 inner coefficients are rebuilt from the row data and this
+  // column's metric products.
   struct Stage {
     T w, l, p, x;
     T zw, zp;
@@ -1005,8 +1015,8 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   };
   T p0 = T(0), pp1 = T(0), dot = T(0);
   auto step = [&](int k, Stage& cur, Stage& nxt) {
-    load_a(k - 2, nxt);
-    load_c(k - 1, nxt);
+    load_a(k - 1 - D, nxt);
+    load_c(k - D, nxt);
     const T pm1 = k > 0 ? level(k - 1, cur) : T(0);
     __syncthreads();
     if (centre) {
@@ -1029,17 +1039,23 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     pp1 = p0;
     p0 = pm1;
   };
-  Stage s0, s1;
+  // set r finishes the steps k = nz-1-r (mod D+1); set D first carries w/l/p_old of the bottom level
+  Stage S[D + 1];
   if (centre) zw_below = tab ? zcol.zw(nz) : zwi[(size_t)nz * P + off];  // ZW(KE+1), the last level's bottom face
-  load_a(nz - 1, s1);
-  load_a(nz - 2, s0);
-  load_c(nz - 1, s0);
-  p0 = level(nz - 1, s1);
+  load_a(nz - 1, S[D]);
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    load_a(nz - 2 - r, S[r]);
+    load_c(nz - 1 - r, S[r]);
+  }
+  p0 = level(nz - 1, S[D]);
   __syncthreads();
-  for (int k = nz - 1; k >= 0; k -= 2) {
-    step(k, s0, s1);
-    if (k == 0) break;
-    step(k - 1, s1, s0);
+  for (int kb = nz - 1; kb >= 0; kb -= D + 1) {
+#pragma unroll
+    for (int r = 0; r <= D; ++r) {
+      if (kb - r < 0) break;
+      step(kb - r, S[r], S[(r + D) % (D + 1)]);
+    }
   }
   publish_rows<T, H, H + 3>(s, t.bx, t.by, dot, T(0));
 }
@@ -1262,9 +1278,9 @@ template <int H>
 static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
   const dim3 grid(ntiles), block(bwd_threads<H>());
   if (fp32)
-    hipLaunchKernelGGL((k_bwd<float, H>), grid, block, 0, st, d_sys, d_tiles, raw);
+    hipLaunchKernelGGL((k_bwd<float, H, bwd_depth<H>()>), grid, block, 0, st, d_sys, d_tiles, raw);
   else
-    hipLaunchKernelGGL((k_bwd<double, H>), grid, block, 0, st, d_sys, d_tiles, raw);
+    hipLaunchKernelGGL((k_bwd<double, H, bwd_depth<H>()>), grid, block, 0, st, d_sys, d_tiles, raw);
   return hipGetLastError();
 }
 

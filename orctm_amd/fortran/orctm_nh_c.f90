@@ -33,7 +33,7 @@ MODULE orctm_nh_c
     INTEGER(C_INT32_T) :: precision, maxit, max_outer, warm_start
     REAL(C_DOUBLE) :: rtol, rtol_inner
     INTEGER(C_INT32_T) :: check_every, device, profile, tile_rows
-    INTEGER(C_INT32_T) :: bottom_solve, reserved(3)
+    INTEGER(C_INT32_T) :: bottom_solve, zw_table, reserved(2)
   END TYPE
 
   TYPE, BIND(C) :: orctm_nh_stats_t

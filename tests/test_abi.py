@@ -56,7 +56,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("no gcc")
     checks = {"orctm_nh_grid_t": (abi.GridC, ["ie", "dlxp", "dzw", "cyclic_x", "open_s"]),
-              "orctm_nh_options_t": (abi.OptionsC, ["precision", "rtol", "check_every", "profile", "tile_rows"]),
+              "orctm_nh_options_t": (abi.OptionsC, ["precision", "rtol", "check_every", "profile", "tile_rows",
+                                                        "bottom_solve", "zw_table"]),
               "orctm_nh_stats_t": (abi.StatsC, ["status", "tile_rows", "rel_res", "total_ms", "zw_table"]),
               "orctm_nh_profile_t": (abi.ProfileC, ["matvec_ms", "cells"]),
               "orctm_nh_server_options_t": (abi.ServerOptionsC, ["hbm_budget", "max_active", "device"]),
