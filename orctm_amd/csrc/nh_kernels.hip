@@ -884,13 +884,12 @@ constexpr int bwd_threads() {
 }
 
 // Levels the matvec kernel's loads run ahead of their use (the depth of its software pipeline).
-// Launches of many tiles hide the latency of a load behind other blocks' work, and one level ahead
-// keeps registers low; the low tiles (4 and 8 rows) serve systems with few tiles, where each block's
-// walk down its columns is a chain of dependent levels and only deeper prefetch overlaps them.
-template <int H>
-constexpr int bwd_depth() {
-  return H == 12 ? 1 : 4;
-}
+// A launch of many tiles hides the latency of a load behind other blocks' work, and one level ahead
+// keeps registers low (FP64 loses a third of its rate at depth 2 or 4 from lost occupancy). A launch
+// of fewer tiles than CUs is a few chains of dependent levels, and there a deeper prefetch overlaps
+// them (512x4x100: 62 -> 58 us per launch).
+constexpr int kDeepDepth = 4;
+constexpr int kDeepTiles = 256;
 
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
 template <typename T, int H, int D>
@@ -1039,22 +1038,36 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
     pp1 = p0;
     p0 = pm1;
   };
-  // set r finishes the steps k = nz-1-r (mod D+1); set D first carries w/l/p_old of the bottom level
-  Stage S[D + 1];
   if (centre) zw_below = tab ? zcol.zw(nz) : zwi[(size_t)nz * P + off];  // ZW(KE+1), the last level's bottom face
-  load_a(nz - 1, S[D]);
+  if constexpr (D == 1) {
+    Stage s0, s1;
+    load_a(nz - 1, s1);
+    load_a(nz - 2, s0);
+    load_c(nz - 1, s0);
+    p0 = level(nz - 1, s1);
+    __syncthreads();
+    for (int k = nz - 1; k >= 0; k -= 2) {
+      step(k, s0, s1);
+      if (k == 0) break;
+      step(k - 1, s1, s0);
+    }
+  } else {
+    // set r finishes the steps k = nz-1-r (mod D+1); set D first carries w/l/p_old of the bottom level
+    Stage S[D + 1];
+    load_a(nz - 1, S[D]);
 #pragma unroll
-  for (int r = 0; r < D; ++r) {
-    load_a(nz - 2 - r, S[r]);
-    load_c(nz - 1 - r, S[r]);
-  }
-  p0 = level(nz - 1, S[D]);
-  __syncthreads();
-  for (int kb = nz - 1; kb >= 0; kb -= D + 1) {
+    for (int r = 0; r < D; ++r) {
+      load_a(nz - 2 - r, S[r]);
+      load_c(nz - 1 - r, S[r]);
+    }
+    p0 = level(nz - 1, S[D]);
+    __syncthreads();
+    for (int kb = nz - 1; kb >= 0; kb -= D + 1) {
 #pragma unroll
-    for (int r = 0; r <= D; ++r) {
-      if (kb - r < 0) break;
-      step(kb - r, S[r], S[(r + D) % (D + 1)]);
+      for (int r = 0; r <= D; ++r) {
+        if (kb - r < 0) break;
+        step(kb - r, S[r], S[(r + D) % (D + 1)]);
+      }
     }
   }
   publish_rows<T, H, H + 3>(s, t.bx, t.by, dot, T(0));
@@ -1274,14 +1287,20 @@ static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, 
   return hipGetLastError();
 }
 
-template <int H>
-static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+template <int H, int D>
+static hipError_t bwd_hd(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
   const dim3 grid(ntiles), block(bwd_threads<H>());
   if (fp32)
-    hipLaunchKernelGGL((k_bwd<float, H, bwd_depth<H>()>), grid, block, 0, st, d_sys, d_tiles, raw);
+    hipLaunchKernelGGL((k_bwd<float, H, D>), grid, block, 0, st, d_sys, d_tiles, raw);
   else
-    hipLaunchKernelGGL((k_bwd<double, H, bwd_depth<H>()>), grid, block, 0, st, d_sys, d_tiles, raw);
+    hipLaunchKernelGGL((k_bwd<double, H, D>), grid, block, 0, st, d_sys, d_tiles, raw);
   return hipGetLastError();
+}
+
+template <int H>
+static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  return ntiles < kDeepTiles ? bwd_hd<H, kDeepDepth>(st, fp32, d_sys, d_tiles, ntiles, raw)
+                             : bwd_hd<H, 1>(st, fp32, d_sys, d_tiles, ntiles, raw);
 }
 
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
