@@ -12,8 +12,8 @@
 ! troneu_device replaces CALL TRONEU (orctm.f90:1414): the barotropic SOR solve on the device.
 ! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
 ! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
-! One GPU holds the whole domain (nprocx = nprocy = 1). Build: gfortran/ifort -c orctm_nh_c.f90 and
-! link -L<repo>/orctm_amd -lorctm_nh.
+! One GPU holds the whole domain (nprocx = nprocy = 1). Build (with the model's -r8 or its equivalent):
+! amdflang|gfortran|ifort -cpp -c orctm_nh_c.f90 [-DCG2D4NONHY], link -L<repo>/orctm_amd -lorctm_nh.
 MODULE orctm_nh_c
   USE, INTRINSIC :: ISO_C_BINDING
   IMPLICIT NONE
@@ -138,11 +138,15 @@ CONTAINS
     TYPE(orctm_nh_grid_t) :: g
     TYPE(orctm_nh_options_t) :: o
     INTEGER(C_INT) :: ierr
+    ! DZW is a plain array in MO_COMMO1 (neither POINTER nor TARGET), so C_LOC may not take it; the
+    ! library reads every grid array during orctm_nh_create only, so a copy that lives for the call serves
+    REAL(C_DOUBLE), TARGET :: dzw_c(KE)
+    dzw_c(:) = DZW(:)
     g%ie = IE; g%je = JE; g%ke = KE
     g%i_start = I_start; g%j_start = J_start
     g%dlxp = C_LOC(DLXP); g%dlyp = C_LOC(DLYP)
     g%dlxu = C_LOC(DLXU); g%dlyv = C_LOC(DLYV)
-    g%ddpo = C_LOC(DDPO); g%weto = C_LOC(WETO); g%dzw = C_LOC(DZW)
+    g%ddpo = C_LOC(DDPO); g%weto = C_LOC(WETO); g%dzw = C_LOC(dzw_c)
     g%cyclic_x = ICYCLI_X; g%cyclic_y = ICYCLI_Y
     g%open_w = MERGE(1, 0, West);  g%open_e = MERGE(1, 0, East)
     g%open_n = MERGE(1, 0, North); g%open_s = MERGE(1, 0, South)
