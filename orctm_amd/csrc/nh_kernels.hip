@@ -885,9 +885,10 @@ constexpr int bwd_threads() {
 
 // Levels the matvec kernel's loads run ahead of their use (the depth of its software pipeline).
 // A launch of many tiles hides the latency of a load behind other blocks' work, and one level ahead
-// keeps registers low (FP64 loses a third of its rate at depth 2 or 4 from lost occupancy). A launch
-// of fewer tiles than CUs is a few chains of dependent levels, and there a deeper prefetch overlaps
-// them (512x4x100: 62 -> 58 us per launch).
+// keeps registers low: 12-row FP64 tiles lose a third of their rate at depth 2 or 4 (occupancy), and
+// FP32 tiles run best at depth 1. A launch of fewer tiles than CUs is a few chains of dependent levels,
+// and there a deeper prefetch overlaps them (512x4x100: 62 -> 58 us per launch). FP64 tiles of 4 and
+// 8 rows, lighter blocks, also gain from it (1024x768x60 FP64 at 8 rows: 19.5 -> 20.5 solves/s).
 constexpr int kDeepDepth = 4;
 constexpr int kDeepTiles = 256;
 
@@ -1299,8 +1300,9 @@ static hipError_t bwd_hd(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
 
 template <int H>
 static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
-  return ntiles < kDeepTiles ? bwd_hd<H, kDeepDepth>(st, fp32, d_sys, d_tiles, ntiles, raw)
-                             : bwd_hd<H, 1>(st, fp32, d_sys, d_tiles, ntiles, raw);
+  const bool deep = ntiles < kDeepTiles || (!fp32 && H < 12);
+  return deep ? bwd_hd<H, kDeepDepth>(st, fp32, d_sys, d_tiles, ntiles, raw)
+              : bwd_hd<H, 1>(st, fp32, d_sys, d_tiles, ntiles, raw);
 }
 
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
