@@ -795,7 +795,8 @@ __global__ __launch_bounds__(NT) void k_refine(const Sys* __restrict__ systems, 
 // The halo lanes of k_bwd run the back-substitution recurrence of their halo columns themselves, so
 // z never goes to memory and no grid-wide step separates the two halves of the line solve.
 
-template <typename T, bool INIT, int H>
+// U: levels per load batch (the batch after next is loaded while one is computed).
+template <typename T, bool INIT, int H, int U>
 __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
@@ -824,7 +825,6 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
     // Batches of U levels in two register sets (ping-pong): the loads of batch b+1 are issued before
     // batch b is computed and stored, so stores (which may alias later loads as far as the compiler
     // knows) never sit between a load and its use, and no prefetched value has to be moved.
-    constexpr int U = 4;
     struct Batch {
       T r[U], q[U], l[U], d[U];
     };
@@ -886,9 +886,10 @@ constexpr int bwd_threads() {
 // Levels the matvec kernel's loads run ahead of their use (the depth of its software pipeline).
 // A launch of many tiles hides the latency of a load behind other blocks' work, and one level ahead
 // keeps registers low: 12-row FP64 tiles lose a third of their rate at depth 2 or 4 (occupancy), and
-// FP32 tiles run best at depth 1. A launch of fewer tiles than CUs is a few chains of dependent levels,
-// and there a deeper prefetch overlaps them (512x4x100: 62 -> 58 us per launch). FP64 tiles of 4 and
-// 8 rows, lighter blocks, also gain from it (1024x768x60 FP64 at 8 rows: 19.5 -> 20.5 solves/s).
+// FP32 tiles run best at depth 1. FP64 tiles of 4 and 8 rows, lighter blocks, gain from 4 levels
+// (1024x768x60 FP64 at 8 rows: 19.5 -> 20.5 solves/s). A launch of fewer tiles than CUs
+// (kDeepTiles) is a few chains of dependent levels: FP32 prefetches 4 levels there, FP64 runs the
+// chunked kernel below, and k_fwd loads longer batches.
 constexpr int kDeepDepth = 4;
 constexpr int kDeepTiles = 256;
 
@@ -1070,6 +1071,164 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
         step(kb - r, S[r], S[(r + D) % (D + 1)]);
       }
     }
+  }
+  publish_rows<T, H, H + 3>(s, t.bx, t.by, dot, T(0));
+}
+
+// The FP64 matvec kernel for launches of fewer tiles than CUs, where each block's walk down its columns
+// is the critical path. The operations are k_bwd's, in k_bwd's order per column (same bits), but the
+// levels go in chunks of CH: each thread first runs the back substitution and search-direction update
+// of its column through the chunk -- one dependent fma per level, the chunk's loads issued together --
+// and writes p to an LDS plane per level; after one barrier the rows of the chunk's levels, which no
+// longer depend on one another, are applied. One barrier per chunk instead of one per level.
+// The LDS ring holds CH+1 planes (a power of two): a chunk's rows read the planes of its own levels
+// and the one above them; p of the level below comes from the row before, in a register.
+template <typename T, int H, int CH>
+__global__ __launch_bounds__(bwd_threads<H>()) void k_bwd_ch(const Sys* __restrict__ systems,
+                                                   const Tile* __restrict__ tiles, int raw) {
+  static_assert(((CH + 1) & CH) == 0, "CH + 1 must be a power of two");
+  constexpr int NS = CH + 1;
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
+  if (in->done != RUN) return;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i0 = t.bx * TX, j0 = t.by * H;
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const size_t P = s.plane, IP = s.pitch;
+  const bool first = in->first != 0;
+  const int pb = in->pbuf;
+  const T beta = in->beta;
+  const bool xupd = !first && !raw;
+  const T alpha = in->alpha;
+  gptr<const T> w = G<const T>(s.z);
+  gptr<const typename Fac<T>::S> L = G<const typename Fac<T>::S>(s.L);
+  gptr<T> xv = G<T>(s.x);
+  gptr<const T> pold = G<const T>(pb ? s.pb : s.pa);
+  gptr<T> pnew = G<T>(pb ? s.pa : s.pb);
+  gptr<T> q = G<T>(s.q);
+  gptr<const T> zwi = G<const T>(s.zw_in);
+  gptr<const T> zpi = G<const T>(s.zp_in);
+  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+
+  __shared__ T lds[NS][H + 2][TX + 2];
+
+  // this thread's slot in the LDS plane and the column it follows (as k_bwd)
+  int gi, gj, sx, sy;
+  bool slot = true;
+  const bool centre = ty < H;
+  if (centre) {
+    gi = i0 + tx;
+    gj = j0 + ty;
+    sx = tx + 1;
+    sy = ty + 1;
+  } else if (ty == H) {
+    gi = i0 + tx;
+    gj = j0 - 1;
+    sx = tx + 1;
+    sy = 0;
+  } else if (ty == H + 1) {
+    gi = i0 + tx;
+    gj = j0 + H;
+    sx = tx + 1;
+    sy = H + 1;
+  } else {
+    slot = tx < 2 * H;
+    const int e = tx < H ? 0 : 1, r = tx < H ? tx : tx - H;
+    gi = e ? i0 + TX : i0 - 1;
+    gj = j0 + (slot ? r : 0);
+    sx = e ? TX + 1 : 0;
+    sy = (slot ? r : 0) + 1;
+  }
+  const bool active = centre && gi < nx && gj < ny;
+  int mi = gi, mj = gj;
+  const bool val = slot && map_ij(mi, mj, nx, ny, s.cyc_x, s.cyc_y);
+  const size_t off = val ? (size_t)mj * IP + mi : 0;
+
+  T cc[7];
+  const bool tab = s.tab != 0;
+  ZCol<T> zcol;
+  if (centre) {
+    gptr<const T> cpi = G<const T>(s.col_in);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) cc[c] = cpi[c * P + off];
+    if (tab)
+      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), P, off);
+  }
+  T zw_below = T(0);
+  if (centre) zw_below = tab ? zcol.zw(nz) : zwi[(size_t)nz * P + off];  // ZW(KE+1), the last level's bottom face
+  T zc = T(0), lc = T(0);  // z and l of this thread's column at the level below
+
+  // back substitution and search-direction update of level m (k_bwd's `level`)
+  auto level = [&](int m, T wv, T lv, T pv, T xv_old) {
+    const T zm = fmad(-lc, zc, wv);
+    const T pm = first ? zm : fmad(beta, pv, zm);
+    zc = zm;
+    lc = raw ? T(0) : lv;
+    if (slot) lds[m & (NS - 1)][sy][sx] = val ? pm : T(0);
+    if (active) {
+      st_nt<T>(pnew, (size_t)m * P + off, pm);
+      if (xupd) st_nt<T>(xv, (size_t)m * P + off, fmad(alpha, pv, xv_old));
+    }
+  };
+  auto ld_x = [&](size_t om) { return (centre && xupd) ? ld_nt<T>(xv, om) : T(0); };
+  {
+    const size_t om = (size_t)(nz - 1) * P + off;
+    level(nz - 1, w[om], Fac<T>::ld(L, om), pold[om], ld_x(om));
+  }
+  T pp1 = T(0), dot = T(0);
+  T aw[CH], al[CH], ap[CH], ax[CH], rzw[CH], rzp[CH];
+  unsigned rf[CH];
+  for (int kh = nz - 1; kh >= 0; kh -= CH) {
+    const int n = kh + 1 < CH ? kh + 1 : CH;  // rows kh .. kh-n+1; back substitution of levels kh-1 .. kh-n
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (u < n) {
+        const int m = kh - 1 - u;
+        const size_t om = (size_t)(m > 0 ? m : 0) * P + off;
+        aw[u] = w[om];
+        al[u] = Fac<T>::ld(L, om);
+        ap[u] = pold[om];
+        ax[u] = ld_x(om);
+        if (centre) {
+          const int k = kh - u;
+          const size_t ck = (size_t)k * P + off;
+          rzw[u] = tab ? zcol.zw(k) : ld_nt<T>(zwi, ck);
+          rzp[u] = tab ? zcol.zp(k) : ld_nt<T>(zpi, ck);
+          rf[u] = ld_nt<uint16_t>(flg, ck);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+      if (u < n && kh - 1 - u >= 0) level(kh - 1 - u, aw[u], al[u], ap[u], ax[u]);
+    __syncthreads();
+    if (centre) {
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (u < n) {
+          const int k = kh - u, b0 = k & (NS - 1);
+          const T p0 = lds[b0][sy][sx];
+          const T pm1 = k > 0 ? lds[(k - 1) & (NS - 1)][sy][sx] : T(0);
+          T c7[7];
+          inner_row<T>(cc, rzw[u], zw_below, rzp[u], rf[u], c7);
+          zw_below = rzw[u];
+          T acc = c7[0] * p0;
+          acc = fmad(c7[1], lds[b0][sy][sx - 1], acc);
+          acc = fmad(c7[2], lds[b0][sy][sx + 1], acc);
+          acc = fmad(c7[3], lds[b0][sy - 1][sx], acc);
+          acc = fmad(c7[4], lds[b0][sy + 1][sx], acc);
+          acc = fmad(c7[5], pm1, acc);
+          acc = fmad(c7[6], pp1, acc);
+          if (active) {
+            st_nt<T>(q, (size_t)k * P + off, acc);
+            dot = fmad(p0, acc, dot);
+          }
+          pp1 = p0;
+        }
+      }
+    }
+    __syncthreads();
   }
   publish_rows<T, H, H + 3>(s, t.bx, t.by, dot, T(0));
 }
@@ -1271,38 +1430,59 @@ hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
   LAUNCH(k_refine<double>, ntiles, st, d_sys, d_tiles);
 }
 
+template <typename T, int H, int U>
+static hipError_t fwd_hu(hipStream_t st, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+  const dim3 grid(ntiles), block(64 * H);
+  if (init)
+    hipLaunchKernelGGL((k_fwd<T, true, H, U>), grid, block, 0, st, d_sys, d_tiles);
+  else
+    hipLaunchKernelGGL((k_fwd<T, false, H, U>), grid, block, 0, st, d_sys, d_tiles);
+  return hipGetLastError();
+}
+
+// Load batches of 4 levels keep many blocks' loads in flight at a time; a launch of fewer tiles than
+// CUs is a few column chains, and longer batches keep each chain's loads a memory round trip ahead of
+// their use: 16 levels (FP32, 4-row tiles) or 8, as many as fit the registers without spilling.
 template <int H>
 static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
-  const dim3 grid(ntiles), block(64 * H);
-  if (fp32) {
-    if (init)
-      hipLaunchKernelGGL((k_fwd<float, true, H>), grid, block, 0, st, d_sys, d_tiles);
-    else
-      hipLaunchKernelGGL((k_fwd<float, false, H>), grid, block, 0, st, d_sys, d_tiles);
-  } else {
-    if (init)
-      hipLaunchKernelGGL((k_fwd<double, true, H>), grid, block, 0, st, d_sys, d_tiles);
-    else
-      hipLaunchKernelGGL((k_fwd<double, false, H>), grid, block, 0, st, d_sys, d_tiles);
-  }
-  return hipGetLastError();
-}
-
-template <int H, int D>
-static hipError_t bwd_hd(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
-  const dim3 grid(ntiles), block(bwd_threads<H>());
+  constexpr int UF = H < 8 ? 16 : 8, UD = H < 12 ? 8 : 4;
+  const bool deep = ntiles < kDeepTiles;
   if (fp32)
-    hipLaunchKernelGGL((k_bwd<float, H, D>), grid, block, 0, st, d_sys, d_tiles, raw);
-  else
-    hipLaunchKernelGGL((k_bwd<double, H, D>), grid, block, 0, st, d_sys, d_tiles, raw);
+    return deep ? fwd_hu<float, H, UF>(st, init, d_sys, d_tiles, ntiles)
+                : fwd_hu<float, H, 4>(st, init, d_sys, d_tiles, ntiles);
+  return deep ? fwd_hu<double, H, UD>(st, init, d_sys, d_tiles, ntiles)
+              : fwd_hu<double, H, 4>(st, init, d_sys, d_tiles, ntiles);
+}
+
+template <typename T, int H, int D>
+static hipError_t bwd_stream(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  hipLaunchKernelGGL((k_bwd<T, H, D>), dim3(ntiles), dim3(bwd_threads<H>()), 0, st, d_sys, d_tiles, raw);
   return hipGetLastError();
 }
 
+template <typename T, int H, int CH>
+static hipError_t bwd_chunked(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  hipLaunchKernelGGL((k_bwd_ch<T, H, CH>), dim3(ntiles), dim3(bwd_threads<H>()), 0, st, d_sys, d_tiles, raw);
+  return hipGetLastError();
+}
+
+// Which matvec kernel a launch runs (all give the same bits), as measured on gfx950:
+//   FP32, 256 tiles or more: k_bwd, 1 level ahead; fewer: k_bwd, 4 levels ahead (512x4x100: 62 -> 58 us;
+//         the chunked form took 76-87 us there for every chunk length tried, 3 to 15)
+//   FP64, 12-row tiles: k_bwd, 1 level ahead; 4 and 8 rows: 4 levels ahead; fewer than 256 tiles:
+//         k_bwd_ch in chunks of 7 levels (4 rows) or 3 (8 and 12 rows, the most that fit the registers
+//         without spilling) (512x4x100: 92 -> 86 us; 128x128x30: 38 -> 34 us)
 template <int H>
 static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
-  const bool deep = ntiles < kDeepTiles || (!fp32 && H < 12);
-  return deep ? bwd_hd<H, kDeepDepth>(st, fp32, d_sys, d_tiles, ntiles, raw)
-              : bwd_hd<H, 1>(st, fp32, d_sys, d_tiles, ntiles, raw);
+  const bool few = ntiles < kDeepTiles;
+  if (fp32)
+    return few ? bwd_stream<float, H, kDeepDepth>(st, d_sys, d_tiles, ntiles, raw)
+               : bwd_stream<float, H, 1>(st, d_sys, d_tiles, ntiles, raw);
+  if (few) return bwd_chunked<double, H, (H < 8 ? 7 : 3)>(st, d_sys, d_tiles, ntiles, raw);
+  if constexpr (H < 12)
+    return bwd_stream<double, H, kDeepDepth>(st, d_sys, d_tiles, ntiles, raw);
+  else
+    return bwd_stream<double, H, 1>(st, d_sys, d_tiles, ntiles, raw);
 }
 
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles) {

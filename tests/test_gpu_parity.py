@@ -286,6 +286,34 @@ def test_tile_height_does_not_change_results(name, precision):
         assert st["outer_iters"] == out[0][1]["outer_iters"]
 
 
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+def test_chunked_and_streaming_inner_kernels_agree(precision):
+    """A launch of fewer tiles than CUs (256) runs the inner kernels in their chunked, latency-bound
+    forms (levels in chunks, one barrier per chunk); a larger one streams a level at a time. On
+    512x132x6, 4-row tiles give 8 x 33 = 264 tiles and 12-row tiles 8 x 11 = 88, so the two forms
+    meet on one system: the same bits for the inner operator, the line solve and the whole solve."""
+    print(f"[chunked vs streaming] precision={precision}", flush=True)
+    g, divg = synth.build(synth.GridSpec(512, 132, 6, smooth=3.0, seed=synth.BASE_SEED + 77))
+    x = synth.normal(5, 3, g.cells)
+    res = {}
+    for rows in (4, 12):
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=precision, tile_rows=rows, warm_start=False)) as s:
+            dt = np.float32 if precision == abi.MIXED else np.float64
+            y = s.inner_apply(x.astype(dt))
+            z, _ = s.inner_precond(x.astype(dt))
+            st = s.solve(divg, pnh)
+        assert st["status"] == "OK" and st["tile_rows"] == rows, st
+        res[rows] = (y, z, g.interior(pnh), st)
+    (y4, z4, p4, s4), (y12, z12, p12, s12) = res[4], res[12]
+    A, S = oracle.assemble(g)
+    dt = np.float32 if precision == abi.MIXED else np.float64
+    assert bits_equal(y4, oracle.matvec(g, oracle.inner_coef(A, S, dt, grid=g), x.astype(dt)))
+    assert bits_equal(y4, y12) and bits_equal(z4, z12)
+    assert bits_equal(p4, p12)
+    assert s4["inner_iters"] == s12["inner_iters"] and s4["outer_iters"] == s12["outer_iters"]
+
+
 def test_batch_with_mixed_tile_heights_matches_alone():
     names = ["regional", "cyclic_x", "one_layer"]
     alone, solvers = {}, []
