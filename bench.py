@@ -38,16 +38,19 @@ MATVEC_BYTES = {4: 36, 8: 72}
 
 # The bytes each kernel itself streams per launch, per cell and per (i, j) column, for word size W
 # (4 FP32, 8 FP64) and line-preconditioner factor size F (2: fp16 in FP32 mode, 8 in FP64 mode).
-# k_bwd reads w, p_old, x (3 words), l (F) and the uint16 coupling flags per cell and writes p, q, x
-#   (3 words); per column the 7 metric products, and either the ZW/ZP window (6 words + an int) when
-#   the system uses the per-level table (stats zw_table = 1) or, per cell, ZW and ZP (2 words) and the
-#   bottom-face ZW per column otherwise.
+# k_bwd reads w, p_old, x (3 words) and l (F) per cell and writes p, q, x (3 words); per column the 7
+#   metric products. With the table form (stats zw_table = 1) it also reads per column the ZW/ZP window
+#   (6 words + an int) and, per cell, the uint16 flags (FP32) or, per column, the wet depths the FP64
+#   kernel derives the flag bits from (3 words of 4 B); otherwise, per cell, ZW, ZP (2 words) and the
+#   flags, and the bottom-face ZW per column.
 # k_fwd (r update + forward elimination + r.z, r.r): reads r, q (2 words), l, invD (2 F), writes r, w;
 #   in FP32 mode also the per-column scale of invD (4 B per column).
 def bwd_bytes(cells, columns, W, table=True):
     F = 2 if W == 4 else 8
     if table:
-        return cells * (6 * W + F + 2) + columns * (13 * W + 4)
+        if W == 4:
+            return cells * (6 * W + F + 2) + columns * (13 * W + 4)
+        return cells * (6 * W + F) + columns * (13 * W + 16)
     return cells * (8 * W + F + 2) + columns * 8 * W
 
 

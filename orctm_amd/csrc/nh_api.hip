@@ -815,6 +815,7 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   ALLOC(tw, (size_t)(h->nz + 1) * sizeof(double), const double*);
   ALLOC(tp, (size_t)h->nz * sizeof(double), const double*);
   ALLOC(xk, (size_t)h->plane * sizeof(int), const int*);
+  ALLOC(xf, 3 * (size_t)h->plane * sizeof(uint32_t), const uint32_t*);
   ALLOC(xw, 3 * (size_t)h->plane * sizeof(double), const double*);
   ALLOC(xp, 3 * (size_t)h->plane * sizeof(double), const double*);
   if (h->fp32) {

@@ -78,6 +78,11 @@ struct Sys {
   const double* xp;     // 3 planes: ZP at levels xk, xk+1, xk+2
   const void* xw_in;    // the same in the inner precision
   const void* xp_in;
+  // per column, 3 words: the wet depth (leading wet levels of WETO) of the column and of its W, E, N, S
+  // neighbours (halo included), 16 bits each, and 4 bits saying whether the W/E/N/S coupling can exist
+  // (periodic or not on the domain edge); with `tab` set the kernels derive each cell's flag bits from
+  // them instead of reading the flag plane (checked bit for bit against it at create)
+  const uint32_t* xf;
   const void* invD;     // line preconditioner 1/D_k: FP64, or fp16 bits scaled by dscale (FP32)
   const void* L;        // line preconditioner l_k: FP64, or fp16 bits (FP32)
   const float* dscale;  // FP32 only: per-column power of two that multiplies the stored invD

@@ -15,6 +15,7 @@
 #include <type_traits>
 
 #include "nh_kernels.h"
+#include "nh_flags.h"
 
 namespace onh {
 
@@ -203,14 +204,17 @@ __device__ __forceinline__ void row_coef(const T* cc, T zwk, T zwk1, T zpk, unsi
   c7[6] = (f & (1u << 6)) ? zwk1 * zpk : T(0);
 }
 
-// ZW/ZP of one column from the per-level table and the column's 3-level window (Sys::tab).
+// ZW/ZP of one column from the per-level table and the column's 3-level window, and the column's flag
+// bits from its wet depths (Sys::tab).
 template <typename T>
 struct ZCol {
   gptr<const T> tw, tp;
   int kx;
   T w0, w1, w2, p0, p1, p2;
+  int kc, kw, ke, kn, ks;
+  unsigned edge;
   __device__ void load(gptr<const T> tw_, gptr<const T> tp_, const int* xk, gptr<const T> xw, gptr<const T> xp,
-                       size_t P, size_t off) {
+                       const uint32_t* xf, size_t P, size_t off) {
     tw = tw_;
     tp = tp_;
     kx = xk[off];
@@ -220,7 +224,16 @@ struct ZCol {
     p0 = xp[off];
     p1 = xp[P + off];
     p2 = xp[2 * P + off];
+    const uint32_t a = xf[off], b = xf[P + off], c = xf[2 * P + off];
+    kc = (int)(a & 0xffffu);
+    kw = (int)(a >> 16);
+    ke = (int)(b & 0xffffu);
+    kn = (int)(b >> 16);
+    ks = (int)(c & 0xffffu);
+    edge = c >> 16;
   }
+  // flag bits of level m (0-based) of a column of nz levels
+  __device__ unsigned flag(int m, int nz) const { return derive_flags(m + 1, nz, kc, kw, ke, kn, ks, edge); }
   __device__ T zw(int m) const {
     const int d = m - kx;
     return d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : tw[m];
@@ -313,6 +326,29 @@ __global__ __launch_bounds__(NT) void k_zwindow(const Sys* __restrict__ systems,
       xpi[c * P + col] = m < nz ? zpi[(size_t)m * P + col] : T(0);
     }
   }
+}
+
+// Per column: the wet depths of the column and of its 4 neighbours from WETO (halo included) and the
+// edge bits, packed into Sys::xf (nh_flags.h); every level's derived flag bits compared with the
+// assembled flag plane. *bad set if any cell differs (a wet cell under a dry one, a WETO value other
+// than 0 or 1).
+__global__ __launch_bounds__(NT) void k_zflags(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
+                                               FortranIn f, int* bad) {
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
+  if (i >= s.nx || j >= s.ny) return;
+  uint32_t w[3];
+  column_flag_words(f.weto, f.IE, f.JE, f.KE, i + 2, j + 2, f.cyc_x, f.cyc_y, w);
+  const size_t P = s.plane, col = (size_t)j * s.pitch + i;
+  uint32_t* xf = const_cast<uint32_t*>(s.xf);
+  xf[col] = w[0];
+  xf[P + col] = w[1];
+  xf[2 * P + col] = w[2];
+  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+  bool diff = false;
+  for (int K = 1; K <= f.KE; ++K) diff = diff || flg[(size_t)(K - 1) * P + col] != flags_from_words(K, f.KE, w);
+  if (diff) atomicOr(bad, 1);
 }
 
 // Assembly into the compact form. ZW/ZP follow mo_petsc.f90:557-590, the per-column products
@@ -690,7 +726,9 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
       gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
       const bool tab = s.tab != 0;
       ZCol<double> zc;
-      if (tab) zc.load(G<const double>(s.tw), G<const double>(s.tp), s.xk, G<const double>(s.xw), G<const double>(s.xp), P, col);
+      if (tab)
+        zc.load(G<const double>(s.tw), G<const double>(s.tp), s.xk, G<const double>(s.xw), G<const double>(s.xp), s.xf,
+                P, col);
       double cc[7];
       load_col(s, col, cc);
       // horizontal neighbour columns (periodic wrap where cyclic; absent ones read this column and
@@ -713,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_residual(const Sys* __restrict__ systems
         v.b = ld_nt<double>(b, col + o);
         v.zp = tab ? zc.zp(kk) : ld_nt<double>(zp, col + o);
         v.zw1 = tab ? zc.zw(kk + 1) : ld_nt<double>(zw, col + o + P);
-        v.f = ld_nt<uint16_t>(flg, col + o);
+        v.f = tab ? zc.flag(kk, nz) : ld_nt<uint16_t>(flg, col + o);
         v.pd = p[col + (kk < nz - 1 ? o + P : o)];
         v.pw = p[cw + o];
         v.pe = p[ce + o];
@@ -893,6 +931,15 @@ constexpr int bwd_threads() {
 constexpr int kDeepDepth = 4;
 constexpr int kDeepTiles = 256;
 
+// With the table form, whether the matvec kernels derive each cell's flag bits from the column's wet
+// depths (2 B per cell less) or read the flag plane: FP64 derives (regional FP64 20.6 -> 20.9 solves/s,
+// batch 177 -> 181); FP32 reads, because the 6 more registers per thread cost its 12-row blocks their
+// second block per CU (mixed 41.3 -> 34.4, or 38.1 when held to 64 registers, with spills). The
+// chunked FP64 kernel of 12-row tiles reads too (it would spill).
+template <typename T>
+constexpr bool kDeriveFlags = sizeof(T) == 8;
+
+
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
 template <typename T, int H, int D>
 __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
@@ -975,7 +1022,8 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
 #pragma unroll
     for (int c = 0; c < 7; ++c) cc[c] = cpi[c * P + off];
     if (tab)
-      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), P, off);
+      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), s.xf, P,
+                off);
   }
   T zw_below = T(0);  // ZW at the bottom face of the current level (top face of the level below)
   (void)NP;
@@ -998,7 +1046,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
       st.zw = ld_nt<T>(zwi, cm);
       st.zp = ld_nt<T>(zpi, cm);
     }
-    st.f = ld_nt<uint16_t>(flg, cm);
+    st.f = (tab && kDeriveFlags<T>) ? zcol.flag(mm, nz) : ld_nt<uint16_t>(flg, cm);
   };
   // back substitution and search-direction update of level m for this thread's column
   auto level = [&](int m, const Stage& st) -> T {
@@ -1153,7 +1201,8 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd_ch(const Sys* __restri
 #pragma unroll
     for (int c = 0; c < 7; ++c) cc[c] = cpi[c * P + off];
     if (tab)
-      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), P, off);
+      zcol.load(G<const T>(s.tw_in), G<const T>(s.tp_in), s.xk, G<const T>(s.xw_in), G<const T>(s.xp_in), s.xf, P,
+                off);
   }
   T zw_below = T(0);
   if (centre) zw_below = tab ? zcol.zw(nz) : zwi[(size_t)nz * P + off];  // ZW(KE+1), the last level's bottom face
@@ -1195,7 +1244,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd_ch(const Sys* __restri
           const size_t ck = (size_t)k * P + off;
           rzw[u] = tab ? zcol.zw(k) : ld_nt<T>(zwi, ck);
           rzp[u] = tab ? zcol.zp(k) : ld_nt<T>(zpi, ck);
-          rf[u] = ld_nt<uint16_t>(flg, ck);
+          rf[u] = (tab && kDeriveFlags<T> && H < 12) ? zcol.flag(k, nz) : ld_nt<uint16_t>(flg, ck);
         }
       }
     }
@@ -1372,8 +1421,14 @@ hipError_t launch_zwindow(hipStream_t st, bool fp32, const Sys* d_sys, const Til
   else hipLaunchKernelGGL(k_ztab<double>, dim3(1), dim3(64), 0, st, d_sys, f);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (fp32) LAUNCH(k_zwindow<float>, ntiles, st, d_sys, d_tiles, d_bad);
-  LAUNCH(k_zwindow<double>, ntiles, st, d_sys, d_tiles, d_bad);
+  if (fp32)
+    hipLaunchKernelGGL(k_zwindow<float>, dim3(ntiles), dim3(NT), 0, st, d_sys, d_tiles, d_bad);
+  else
+    hipLaunchKernelGGL(k_zwindow<double>, dim3(ntiles), dim3(NT), 0, st, d_sys, d_tiles, d_bad);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_zflags, dim3(ntiles), dim3(NT), 0, st, d_sys, d_tiles, f, d_bad);
+  return hipGetLastError();
 }
 
 hipError_t launch_line_factor(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles) {

@@ -3,8 +3,10 @@ mo_petsc.f90:557-590): on the shared geometries, whose partial cells sit at the 
 boden.f90:182-213 builds them, every system takes the table form when asked (options zw_table = 1, stats
 zw_table = 1) and stays bit-exact (inner operator) and within the solve tolerances; a grid with a partial
 cell in the middle of a column does not fit the window, falls back to the per-cell planes (zw_table = 0),
-and is just as exact. By default (zw_table = 0) the table is taken only by launches of at least 2^20
-cells, and a solve gives the same bits whichever form it reads.
+and is just as exact. The table form also derives each cell's flag bits from the wet depths of its
+column and of the 4 neighbours (checked against the assembled flags at create): a wet cell under a dry
+one falls back the same way. By default (zw_table = 0) the table is taken only by launches of at least
+2^20 cells, and a solve gives the same bits whichever form it reads.
 """
 import copy
 
@@ -66,6 +68,20 @@ def test_mid_column_partial_cell_falls_back(precision):
     ddpo[i, j, 3] = 0.5 * ddpo[i, j, 3]  # a partial cell four levels above the bottom region
     g.ddpo = ddpo
     g._c = None  # the C view of the grid is built from the arrays on first use
+    st = check(g, divg, precision)
+    assert st["zw_table"] == 0
+
+
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+def test_wet_cell_under_dry_cell_falls_back(precision):
+    g0, divg = build("regional")
+    g = copy.copy(g0)
+    weto = g0.weto.copy(order="F")
+    deep = np.argwhere(g0.weto[1:-1, 1:-1, :].sum(axis=2) >= 6)[0] + 1
+    i, j = int(deep[0]), int(deep[1])
+    weto[i, j, 2] = 0.0  # a dry cell at level 3 above wet ones: not a z-level column
+    g.weto = weto
+    g._c = None
     st = check(g, divg, precision)
     assert st["zw_table"] == 0
 
