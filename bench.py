@@ -31,7 +31,7 @@ from orctm_amd.nh import NHSolver, options  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# The roofline figure of the hot kernel is the survey's matvec y = A x: 7 coefficients + x + y per cell of
+# The roofline figure of the hot kernel is the bare matvec y = A x: 7 coefficients + x + y per cell of
 # the box, 36 B in FP32 (72 B in FP64). k_bwd performs that matvec (and the back substitution, the
 # search-direction update, the deferred x update and p.q) per launch.
 MATVEC_BYTES = {4: 36, 8: 72}
@@ -237,7 +237,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
                      "bytes_per_launch": mv_bytes, "bytes_per_cell": MATVEC_BYTES[4],
-                     "bytes_note": "the survey's matvec: 7 coefficients + x + y per cell (FP32)",
+                     "bytes_note": "the bare matvec y = A x: 7 coefficients + x + y per cell (FP32)",
                      "kernel_bytes_per_launch": kern_bytes, "kernel_GBps": round(kern_bytes / (mv_avg_ms * 1e-3) / 1e9, 1),
                      "zw_table": int(table), "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
         "other_kernels": {
@@ -351,7 +351,7 @@ def bench_batch(args):
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "the survey's matvec bytes (36 B per cell, FP32) per active cell-iteration over the "
+                     "note": "the bare matvec bytes (36 B per cell, FP32) per active cell-iteration over the "
                              "summed matvec launch time"},
         "cpu_baseline": None,
     }

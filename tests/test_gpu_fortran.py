@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 
 def python_path(g, divg, v, b, b1o, cg2d):
-    """The driver's calls through the Python mirror: petsc_start/solve_3D/ocvtot/troneu/petsc_end."""
+    """The Fortran host's calls through the Python mirror: petsc_start/solve_3D/ocvtot/troneu/petsc_end."""
     out = []
     with NHSolver(g, options(bottom_solve=cg2d)) as s:
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
