@@ -368,6 +368,8 @@ def _bot_lib():
         L.orc_op2d_apply.argtypes = [ctypes.POINTER(GridC), _dp, _dp, _dp]
         L.orc_solve2d.argtypes = [ctypes.POINTER(GridC), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
         L.orc_solve2d.restype = ctypes.c_int
+        L.orc_mg_apply.argtypes = [ctypes.POINTER(GridC), _dp, _dp, _dp]
+        L.orc_mg_apply.restype = ctypes.c_int
         L.orc_bottom_correction.argtypes = [ctypes.POINTER(GridC), _dp, _dp]
         L._bot = True
     return L
@@ -422,13 +424,22 @@ def op2d_scipy(grid, coef):
 
 
 def solve2d(grid, coef, b, x0=None, rtol=1e-10, maxit=100000):
-    """The device's 2-D PCG restated on the CPU: (x, iterations, rel_res)."""
+    """The device's 2-D PCG (V-cycle preconditioned) restated on the CPU: (x, iterations, rel_res)."""
     L = _bot_lib()
     x = np.zeros(grid.nx * grid.ny) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     rr = ctypes.c_double()
     it = L.orc_solve2d(ctypes.byref(grid.c_struct()), _d(np.ascontiguousarray(coef)), _d(np.ascontiguousarray(b)),
                        _d(x), rtol, maxit, ctypes.byref(rr))
     return x, it, rr.value
+
+
+def mg_apply(grid, coef, r):
+    """One V-cycle of the 2-D solve's multigrid preconditioner: (z, levels)."""
+    L = _bot_lib()
+    z = np.zeros(grid.nx * grid.ny)
+    nl = L.orc_mg_apply(ctypes.byref(grid.c_struct()), _d(np.ascontiguousarray(coef)),
+                        _d(np.ascontiguousarray(r, dtype=np.float64)), _d(z))
+    return z, nl
 
 
 def bottom_correction(grid, pnh_bot, divg):

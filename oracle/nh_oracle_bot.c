@@ -7,8 +7,8 @@
  *   orc_bottom_rhs         ocvtot_nh.f90:64-144   bottom values of GU, GV, GW, their divergence,
  *                                                 DIVG_bot_surf
  *   orc_op2d_coef/_apply   mo_petsc.f90:401-538   compute_matrix_2d with Neumann = .FALSE. (:37)
- *   orc_solve2d            the device's PCG (no counterpart in the reference: its cg2d_ KSP runs
- *                          inside PETSc, mo_petsc.f90:147-166)
+ *   orc_solve2d            the device's PCG with its multigrid V-cycle (no counterpart in the
+ *                          reference: its cg2d_ KSP runs inside PETSc, mo_petsc.f90:147-166)
  *   orc_bottom_correction  mo_petsc.f90:252-374   ComputeRHS's corrections of DIVG by PNH_bot
  *
  * Every expression is evaluated left to right as the Fortran source writes it (no contraction).
@@ -132,6 +132,245 @@ static double norm2(const double* v, size_t n) {
   return sqrt(s);
 }
 
+/* The multigrid preconditioner: level 0 is the box (unknowns i in [1, nx-2], j in [1, ny-2]) with
+ * K = -S*A2 on them (couplings to the box edge dropped); each next level's cell is a 2 x 2 aggregate of
+ * the unknowns before it, with the Galerkin operator P^T K P of the piecewise-constant P (a coupling
+ * inside an aggregate adds to the coarse centre, one across a side to that side's coarse coupling);
+ * the last level (at most MG_DENSE unknowns) is inverted by Gauss-Jordan with partial pivoting. */
+#define MG_DENSE 64
+#define MG_OMEGA (2.0 / 3.0)
+#define MG_ALPHA 1.8
+#define MG_NU 2
+#define MG_CHECK 8
+
+typedef struct {
+  int a, b, o;
+  double *k, *x, *y, *r;
+} mglev;
+
+static int mg_active(const mglev* L, int i, int j) { return i >= L->o && j >= L->o && i < L->a - L->o && j < L->b - L->o; }
+
+static double mg_kx(const mglev* L, const double* x, int i, int j, size_t c) {
+  const size_t n = (size_t)L->a * L->b;
+  double s = L->k[c] * x[c];
+  if (i > 0) s += L->k[n + c] * x[c - 1];
+  if (i < L->a - 1) s += L->k[2 * n + c] * x[c + 1];
+  if (j > 0) s += L->k[3 * n + c] * x[c - L->a];
+  if (j < L->b - 1) s += L->k[4 * n + c] * x[c + L->a];
+  return s;
+}
+
+static long mg_unknowns(const mglev* L) { return (long)(L->a - 2 * L->o) * (L->b - 2 * L->o); }
+
+/* y = x + omega (r - K x) / K_cc on the unknowns (x = 0 when x is NULL), 0 elsewhere */
+static void mg_jac(const mglev* L, const double* r, const double* x, double* y) {
+  for (int j = 0; j < L->b; ++j)
+    for (int i = 0; i < L->a; ++i) {
+      const size_t c = (size_t)j * L->a + i;
+      if (!mg_active(L, i, j))
+        y[c] = 0.0;
+      else if (!x)
+        y[c] = MG_OMEGA * r[c] / L->k[c];
+      else
+        y[c] = x[c] + MG_OMEGA * (r[c] - mg_kx(L, x, i, j, c)) / L->k[c];
+    }
+}
+
+typedef struct {
+  int nl, m;
+  mglev* lv;
+  double* inv;
+} mghier;
+
+static void mg_free(mghier* h) {
+  for (int l = 0; l < h->nl; ++l) {
+    free(h->lv[l].k);
+    free(h->lv[l].x);
+    free(h->lv[l].y);
+    free(h->lv[l].r);
+  }
+  free(h->lv);
+  free(h->inv);
+}
+
+static int mg_build(int nx, int ny, const double* coef, const double* S, mghier* h) {
+  const size_t n = (size_t)nx * ny;
+  const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+  h->nl = 1;
+  h->lv = (mglev*)calloc(64, sizeof(mglev));
+  mglev* L0 = &h->lv[0];
+  L0->a = nx;
+  L0->b = ny;
+  L0->o = 1;
+  L0->k = (double*)calloc(5 * n, sizeof(double));
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      if (!mg_active(L0, i, j)) {
+        L0->k[c] = 1.0;
+        continue;
+      }
+      L0->k[c] = -S[c] * coef[c];
+      for (int p = 1; p < 5; ++p)
+        if (mg_active(L0, i + di[p], j + dj[p])) L0->k[p * n + c] = -S[c] * coef[p * n + c];
+    }
+  while (mg_unknowns(&h->lv[h->nl - 1]) > MG_DENSE) {
+    const mglev* F = &h->lv[h->nl - 1];
+    mglev* C = &h->lv[h->nl];
+    const int wa = F->a - 2 * F->o, wb = F->b - 2 * F->o;
+    C->a = (wa + 1) / 2;
+    C->b = (wb + 1) / 2;
+    C->o = 0;
+    const size_t fn = (size_t)F->a * F->b, cn = (size_t)C->a * C->b;
+    C->k = (double*)calloc(5 * cn, sizeof(double));
+    for (int j = F->o; j < F->b - F->o; ++j)
+      for (int i = F->o; i < F->a - F->o; ++i) {
+        const size_t f = (size_t)j * F->a + i;
+        const int I = (i - F->o) / 2, J = (j - F->o) / 2;
+        const size_t cc = (size_t)J * C->a + I;
+        C->k[cc] += F->k[f];
+        for (int p = 1; p < 5; ++p) {
+          const int ni = i + di[p], nj = j + dj[p];
+          if (!mg_active(F, ni, nj)) continue;
+          if ((ni - F->o) / 2 == I && (nj - F->o) / 2 == J)
+            C->k[cc] += F->k[p * fn + f];
+          else
+            C->k[p * cn + cc] += F->k[p * fn + f];
+        }
+      }
+    h->nl++;
+  }
+  for (int l = 0; l < h->nl; ++l) {
+    const size_t cnt = (size_t)h->lv[l].a * h->lv[l].b;
+    h->lv[l].x = (double*)calloc(cnt, sizeof(double));
+    h->lv[l].y = (double*)calloc(cnt, sizeof(double));
+    h->lv[l].r = (double*)calloc(cnt, sizeof(double));
+  }
+  const mglev* T = &h->lv[h->nl - 1];
+  const int wa = T->a - 2 * T->o, m = (int)mg_unknowns(T);
+  const size_t tn = (size_t)T->a * T->b;
+  double* M = (double*)calloc((size_t)m * m, sizeof(double));
+  h->inv = (double*)calloc((size_t)m * m, sizeof(double));
+  h->m = m;
+  for (int t = 0; t < m; ++t) {
+    const int i = t % wa + T->o, j = t / wa + T->o;
+    const size_t c = (size_t)j * T->a + i;
+    for (int p = 0; p < 5; ++p) {
+      const int ni = i + di[p], nj = j + dj[p];
+      if (!mg_active(T, ni, nj)) continue;
+      M[(size_t)t * m + (size_t)(nj - T->o) * wa + (ni - T->o)] += T->k[p * tn + c];
+    }
+    h->inv[(size_t)t * m + t] = 1.0;
+  }
+  int ok = 1;
+  for (int col = 0; col < m && ok; ++col) {
+    int piv = col;
+    for (int rr = col + 1; rr < m; ++rr)
+      if (fabs(M[(size_t)rr * m + col]) > fabs(M[(size_t)piv * m + col])) piv = rr;
+    if (!(fabs(M[(size_t)piv * m + col]) > 0.0)) {
+      ok = 0;
+      break;
+    }
+    if (piv != col)
+      for (int q = 0; q < m; ++q) {
+        double t1 = M[(size_t)piv * m + q];
+        M[(size_t)piv * m + q] = M[(size_t)col * m + q];
+        M[(size_t)col * m + q] = t1;
+        t1 = h->inv[(size_t)piv * m + q];
+        h->inv[(size_t)piv * m + q] = h->inv[(size_t)col * m + q];
+        h->inv[(size_t)col * m + q] = t1;
+      }
+    const double d = M[(size_t)col * m + col];
+    for (int q = 0; q < m; ++q) {
+      M[(size_t)col * m + q] /= d;
+      h->inv[(size_t)col * m + q] /= d;
+    }
+    for (int rr = 0; rr < m; ++rr) {
+      if (rr == col) continue;
+      const double f = M[(size_t)rr * m + col];
+      if (f == 0.0) continue;
+      for (int q = 0; q < m; ++q) {
+        M[(size_t)rr * m + q] -= f * M[(size_t)col * m + q];
+        h->inv[(size_t)rr * m + q] -= f * h->inv[(size_t)col * m + q];
+      }
+    }
+  }
+  free(M);
+  return ok;
+}
+
+/* z = V-cycle(r) on level l: kNu damped-Jacobi sweeps, the residual summed over each aggregate, the
+ * coarse solution scaled by MG_ALPHA added back, kNu sweeps; the dense inverse on the last level */
+static void mg_vcycle(mghier* h, int l, const double* r, double* z) {
+  mglev* L = &h->lv[l];
+  const size_t cnt = (size_t)L->a * L->b;
+  if (l == h->nl - 1) {
+    const int wa = L->a - 2 * L->o, m = h->m;
+    for (int j = 0; j < L->b; ++j)
+      for (int i = 0; i < L->a; ++i) {
+        const size_t c = (size_t)j * L->a + i;
+        double s = 0.0;
+        if (mg_active(L, i, j)) {
+          const int t = (j - L->o) * wa + (i - L->o);
+          for (int q = 0; q < m; ++q) s += h->inv[(size_t)t * m + q] * r[(size_t)(q / wa + L->o) * L->a + (q % wa + L->o)];
+        }
+        z[c] = s;
+      }
+    return;
+  }
+  double *xa = L->x, *xb = L->y;
+  mg_jac(L, r, NULL, xa);
+  for (int s = 1; s < MG_NU; ++s) {
+    mg_jac(L, r, xa, xb);
+    double* t = xa;
+    xa = xb;
+    xb = t;
+  }
+  mglev* C = &h->lv[l + 1];
+  for (int J = 0; J < C->b; ++J)
+    for (int I = 0; I < C->a; ++I) {
+      double s = 0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int i = L->o + 2 * I + di, j = L->o + 2 * J + dj;
+          if (mg_active(L, i, j)) {
+            const size_t f = (size_t)j * L->a + i;
+            s += r[f] - mg_kx(L, xa, i, j, f);
+          }
+        }
+      C->r[(size_t)J * C->a + I] = s;
+    }
+  double* zc = (double*)calloc((size_t)C->a * C->b, sizeof(double));
+  mg_vcycle(h, l + 1, C->r, zc);
+  for (int j = L->o; j < L->b - L->o; ++j)
+    for (int i = L->o; i < L->a - L->o; ++i) xa[(size_t)j * L->a + i] += MG_ALPHA * zc[(size_t)((j - L->o) / 2) * C->a + (i - L->o) / 2];
+  free(zc);
+  for (int s = 1; s <= MG_NU; ++s) {
+    double* out = (s == MG_NU) ? z : xb;
+    mg_jac(L, r, xa, out);
+    xb = xa;
+    xa = out;
+  }
+  (void)cnt;
+}
+
+int orc_mg_apply(const orctm_nh_grid_t* g, const double* coef, const double* r, double* z) {
+  const int IE = g->ie, JE = g->je, nx = IE - 2, ny = JE - 2;
+  const size_t n = (size_t)nx * ny;
+  if (nx <= 2 || ny <= 2) return 0;
+  double* S = (double*)calloc(n, sizeof(double));
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) S[(size_t)i + (size_t)nx * j] = g->dlxp[P2(i + 2, j + 2)] * g->dlyp[P2(i + 2, j + 2)];
+  mghier h;
+  memset(&h, 0, sizeof(h));
+  const int ok = mg_build(nx, ny, coef, S, &h);
+  if (ok) mg_vcycle(&h, 0, r, z);
+  const int nl = h.nl;
+  mg_free(&h);
+  free(S);
+  return ok ? nl : 0;
+}
+
 int orc_solve2d(const orctm_nh_grid_t* g, const double* coef, const double* b, double* x, double rtol, int maxit,
                 double* rel_res) {
   const int IE = g->ie, JE = g->je, nx = IE - 2, ny = JE - 2;
@@ -151,13 +390,15 @@ int orc_solve2d(const orctm_nh_grid_t* g, const double* coef, const double* b, d
   for (size_t c = 0; c < n; ++c) q[c] = b[c] - ax[c];
   double rr = norm2(q, n);
   int it = 0;
-  if (rr > rtol * bn && nx > 2 && ny > 2) {
+  mghier h;
+  memset(&h, 0, sizeof(h));
+  if (rr > rtol * bn && nx > 2 && ny > 2 && mg_build(nx, ny, coef, S, &h)) {
     /* PCG on the inner rows of C = -S*A (symmetric when DLYP is constant along i and DLXP along j),
-       Jacobi preconditioner diag(C) = -S*A_cc; the box-edge values enter through the residual */
+       preconditioned by one V-cycle; the box-edge values enter through the residual */
+    for (size_t c = 0; c < n; ++c) r[c] = inner[c] != 0.0 ? -S[c] * q[c] : 0.0;
+    mg_vcycle(&h, 0, r, z);
     double rz = 0.0;
     for (size_t c = 0; c < n; ++c) {
-      r[c] = inner[c] != 0.0 ? -S[c] * q[c] : 0.0;
-      z[c] = inner[c] != 0.0 ? r[c] / (-S[c] * coef[c]) : 0.0;
       d[c] = z[c];
       rz += r[c] * z[c];
     }
@@ -170,15 +411,16 @@ int orc_solve2d(const orctm_nh_grid_t* g, const double* coef, const double* b, d
       }
       if (!(dq > 0.0)) break;
       const double alpha = rz / dq;
-      double rzn = 0.0;
       for (size_t c = 0; c < n; ++c) {
+        if (inner[c] == 0.0) continue;
         x[c] += alpha * d[c];
         r[c] -= alpha * q[c];
-        z[c] = inner[c] != 0.0 ? r[c] / (-S[c] * coef[c]) : 0.0;
-        rzn += r[c] * z[c];
       }
+      mg_vcycle(&h, 0, r, z);
+      double rzn = 0.0;
+      for (size_t c = 0; c < n; ++c) rzn += r[c] * z[c];
       ++it;
-      if (it % 32 == 0 || it == maxit) {
+      if (it % MG_CHECK == 0 || it == maxit) {
         orc_op2d_apply(g, coef, x, ax);
         for (size_t c = 0; c < n; ++c) q[c] = b[c] - ax[c];
         rr = norm2(q, n);
@@ -192,6 +434,7 @@ int orc_solve2d(const orctm_nh_grid_t* g, const double* coef, const double* b, d
     for (size_t c = 0; c < n; ++c) q[c] = b[c] - ax[c];
     rr = norm2(q, n);
   }
+  if (h.lv) mg_free(&h);
   if (rel_res) *rel_res = bn > 0.0 ? rr / bn : rr;
   free(w);
   return it;

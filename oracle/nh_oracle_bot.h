@@ -24,11 +24,17 @@ void orc_op2d_coef(const orctm_nh_grid_t* g, double* coef);
 void orc_op2d_apply(const orctm_nh_grid_t* g, const double* coef, const double* p, double* y);
 
 /* The 2-D solve as the device runs it, in FP64 on the CPU (no counterpart in the reference, whose cg2d_
- * KSP runs inside PETSc): the box-edge rows directly (p = b / A_cc), then Jacobi-preconditioned CG on the
- * remaining rows scaled by S = DLXP*DLYP, from x (the warm start, edge values ignored), checking
- * ||b - A2 p||_2 <= rtol ||b||_2 every 32 iterations. Returns the iterations; *rel_res the final ratio. */
+ * KSP runs inside PETSc): the box-edge rows directly (p = b / A_cc), then CG on the remaining rows scaled
+ * by S = DLXP*DLYP, preconditioned by one multigrid V-cycle (2 x 2 aggregates, Galerkin operators, 2
+ * damped-Jacobi sweeps before and after each coarse correction, corrections scaled by 1.8, a dense solve
+ * on the last level of at most 64 unknowns), from x (the warm start, edge values ignored), checking
+ * ||b - A2 p||_2 <= rtol ||b||_2 every 8 iterations. Returns the iterations; *rel_res the final ratio. */
 int orc_solve2d(const orctm_nh_grid_t* g, const double* coef, const double* b, double* x, double rtol, int maxit,
                 double* rel_res);
+
+/* One V-cycle of that preconditioner, z = M r, on the box (r on the box edge ignored, z 0 there).
+ * Returns the number of levels (0 if the box has no inner rows). */
+int orc_mg_apply(const orctm_nh_grid_t* g, const double* coef, const double* r, double* z);
 
 /* ComputeRHS with CG2D4NONHY (mo_petsc.f90:252-374): the corrections of DIVG on i = 2..IE-1,
  * j = 2..JE-1, k = 1..KE by PNH_bot (IE,JE, halo included) at dry neighbours and the bottom face, in place. */

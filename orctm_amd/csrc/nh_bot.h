@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "nh_ocv.h"
 
 namespace onh {
@@ -18,6 +20,18 @@ struct BotState {
   double *dbs = nullptr, *pnh_bot = nullptr;                      // DIVG_bot_surf, PNH_bot (IE,JE)
   double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
   double* part = nullptr;  // block partial sums, 3 per block
+  // Multigrid preconditioner of the 2-D PCG, built by bot_setup: level 0 is the box (its edge rows are
+  // not unknowns), each next level aggregates 2 x 2 cells of the one before with the Galerkin operator
+  // (again 5-point), and the last level (at most kMgDense unknowns) is solved with a dense inverse.
+  struct MgLevel {
+    int a = 0, b = 0, o = 0;  // dimensions; the unknowns are i in [o, a-o), j in [o, b-o)
+    double* k = nullptr;      // 5 planes of a*b: centre, west, east, north (j-1), south (j+1)
+    double *x = nullptr, *y = nullptr, *r = nullptr;  // iterate pair and right-hand side
+  };
+  std::vector<MgLevel> mg;
+  double* mg_inv = nullptr;  // inverse of the last level's operator over its unknowns, row-major
+  int mg_n = 0;
+  std::vector<void*> mg_allocs;
   double* scal = nullptr;  // rz, dq, alpha, beta, rr, bb, breakdown
   double* scal_host = nullptr;  // pinned copy of scal
   int nblk = 0;
@@ -28,11 +42,12 @@ struct BotState {
   int status = 0;
 };
 
-constexpr int kBotChunk = 32;
+constexpr int kBotChunk = 8;   // PCG iterations per captured graph, between residual checks
+constexpr int kMgDense = 64;   // unknowns of the last multigrid level at most
 
-// coef and S from the metrics (mo_petsc.f90:401-538). The caller allocates every array of BotState
-// (zero-filled: the module arrays start at 0 as beleg_zero.f90:481-504 sets them), scal_host pinned
-// and the two events.
+// coef and S from the metrics (mo_petsc.f90:401-538) and the multigrid hierarchy (allocated here,
+// freed by bot_release). The caller allocates every other array of BotState (zero-filled: the module
+// arrays start at 0 as beleg_zero.f90:481-504 sets them), scal_host pinned and the two events.
 hipError_t bot_setup(hipStream_t st, BotState& b);
 // ocvtot_nh.f90:64-147 and petsc_solve_2D (mo_petsc.f90:147-166): BOTGU/BOTGV/BOTGW, DIVG_bot_surf from
 // the velocity state and DIVG (first level, interior) of this step, then PNH_bot from the 2-D solve

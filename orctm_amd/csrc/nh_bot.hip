@@ -7,11 +7,13 @@
 //                   source's K loop assigns it (one thread per column walks K)
 //   k_bot_rhs       DIVG_bot_surf = bottom divergence - DIVG(:,:,1)*WETO on the interior
 //   2-D solve       compute_matrix_2d's operator on the (IE-2) x (JE-2) interior box: the box-edge
-//                   rows (diagonal only, Neumann = .FALSE.) directly, then Jacobi-preconditioned CG on
-//                   the remaining rows scaled by S = DLXP*DLYP, warm-started from the previous
-//                   solution as KSPSetInitialGuessNonzero does (mo_petsc.f90:165), to
-//                   ||b - A2 p|| <= rtol ||b|| checked every kBotChunk iterations; block sums in fixed
-//                   order, so the iterates are the same bits on every run
+//                   rows (diagonal only, Neumann = .FALSE.) directly, then CG on the remaining rows
+//                   scaled by S = DLXP*DLYP, preconditioned by one multigrid V-cycle (2 x 2 aggregates,
+//                   Galerkin operators, damped Jacobi smoothing, a dense solve on the last level),
+//                   warm-started from the previous solution as KSPSetInitialGuessNonzero does
+//                   (mo_petsc.f90:165), to ||b - A2 p|| <= rtol ||b|| checked every kBotChunk
+//                   iterations; block sums in fixed order and element-wise V-cycle kernels, so the
+//                   iterates are the same bits on every run
 //   k2_scatter      PNH_bot on the interior, then bounds_exch(PNH_bot) (ocvtot_nh.f90:146)
 //   k_bot_correct   ComputeRHS's corrections of DIVG by PNH_bot at dry neighbours and the bottom face
 // FP64 throughout; the bottom values, DIVG_bot_surf and the corrections evaluate each expression left
@@ -19,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <vector>
 
 #include "../../include/orctm_nh.h"
 #include "nh_bot.h"
@@ -201,15 +204,15 @@ __global__ __launch_bounds__(BB) void k2_start(BDims g, const double* __restrict
   d1[c] = 0.0;
 }
 
-// t = b - A2 x on every row; partial sums of t^2 and b^2; with init also r = -S t, z = r / (-S A_cc) on the
-// inner rows and the partial sum of r z.
+// t = b - A2 x on every row; partial sums of t^2 and b^2; with init also r = -S t on the inner rows
+// (0 on the edge), which the V-cycle then preconditions.
 __global__ __launch_bounds__(BB) void k2_resid(BDims g, const double* __restrict__ coef,
                                                const double* __restrict__ S, const double* __restrict__ b,
                                                const double* __restrict__ x, double* __restrict__ r,
-                                               double* __restrict__ z, double* __restrict__ part, int init) {
+                                               double* __restrict__ part, int init) {
   const int c = blockIdx.x * BB + threadIdx.x;
   const int n = g.nx * g.ny;
-  double tt = 0.0, bb = 0.0, rz = 0.0;
+  double tt = 0.0, bb = 0.0;
   if (c < n) {
     double ax;
     const bool in = g.inner(c);
@@ -218,15 +221,9 @@ __global__ __launch_bounds__(BB) void k2_resid(BDims g, const double* __restrict
     const double t = b[c] - ax;
     tt = t * t;
     bb = b[c] * b[c];
-    if (init) {
-      const double rc = in ? -S[c] * t : 0.0;
-      const double zc = in ? rc / (-S[c] * coef[c]) : 0.0;
-      r[c] = rc;
-      z[c] = zc;
-      rz = rc * zc;
-    }
+    if (init) r[c] = in ? -S[c] * t : 0.0;
   }
-  block_sums(tt, bb, rz, part);
+  block_sums(tt, bb, 0.0, part);
 }
 
 // d = z + beta d_old on the inner rows (zero on the edge), q = -S A2 d, partial sum of d q. The
@@ -257,27 +254,117 @@ __global__ __launch_bounds__(BB) void k2_dirmv(BDims g, const double* __restrict
   block_sums(dq, 0.0, 0.0, part);
 }
 
-// x += alpha d, r -= alpha q, z = r / (-S A_cc) on the inner rows; partial sum of r z.
-__global__ __launch_bounds__(BB) void k2_update(BDims g, const double* __restrict__ coef,
-                                                const double* __restrict__ S, double* __restrict__ x,
-                                                const double* __restrict__ d, double* __restrict__ r,
-                                                const double* __restrict__ q, double* __restrict__ z,
-                                                double* __restrict__ part, const double* __restrict__ scal) {
+// x += alpha d, r -= alpha q on the inner rows (the V-cycle then forms z and r.z).
+__global__ __launch_bounds__(BB) void k2_update(BDims g, double* __restrict__ x, const double* __restrict__ d,
+                                                double* __restrict__ r, const double* __restrict__ q,
+                                                const double* __restrict__ scal) {
   const int c = blockIdx.x * BB + threadIdx.x;
   const double alpha = scal[S_ALPHA];
-  double rz = 0.0;
   if (c < g.nx * g.ny && g.inner(c)) {
     x[c] += alpha * d[c];
-    const double rc = r[c] - alpha * q[c];
-    const double zc = rc / (-S[c] * coef[c]);
-    r[c] = rc;
-    z[c] = zc;
-    rz = rc * zc;
+    r[c] = r[c] - alpha * q[c];
   }
-  block_sums(rz, 0.0, 0.0, part);
 }
 
-enum { R_INIT = 0, R_ALPHA, R_BETA, R_CHECK };
+// ---- the multigrid V-cycle (element-wise kernels; the host issues them level by level) ------------
+
+constexpr double kOmega = 2.0 / 3.0;  // damped Jacobi
+constexpr double kAlpha = 1.8;        // scale of every coarse-grid correction
+constexpr int kNu = 2;                // Jacobi sweeps before and after each coarse correction
+
+struct MgDev {
+  int a, b, o;
+  const double* k;
+  __device__ bool active(int i, int j) const { return i >= o && j >= o && i < a - o && j < b - o; }
+  __device__ double kx(const double* __restrict__ x, int i, int j, int c) const {
+    const size_t n = (size_t)a * b;
+    double s = k[c] * x[c];
+    if (i > 0) s += k[n + c] * x[c - 1];
+    if (i < a - 1) s += k[2 * n + c] * x[c + 1];
+    if (j > 0) s += k[3 * n + c] * x[c - a];
+    if (j < b - 1) s += k[4 * n + c] * x[c + a];
+    return s;
+  }
+};
+
+// x = omega r / K_cc on the unknowns, 0 elsewhere (the first sweep from x = 0)
+__global__ __launch_bounds__(BB) void k_mg_jac0(MgDev L, const double* __restrict__ r, double* __restrict__ x) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  if (c >= L.a * L.b) return;
+  const int i = c % L.a, j = c / L.a;
+  x[c] = L.active(i, j) ? kOmega * r[c] / L.k[c] : 0.0;
+}
+
+// y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere; with dot, the block sums of r y
+__global__ __launch_bounds__(BB) void k_mg_jac(MgDev L, const double* __restrict__ r, const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part, int dot) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  double ry = 0.0;
+  if (c < L.a * L.b) {
+    const int i = c % L.a, j = c / L.a;
+    double v = 0.0;
+    if (L.active(i, j)) {
+      v = x[c] + kOmega * (r[c] - L.kx(x, i, j, c)) / L.k[c];
+      ry = r[c] * v;
+    }
+    y[c] = v;
+  }
+  if (dot) block_sums(ry, 0.0, 0.0, part);
+}
+
+// rc = the sum of r - K x over each aggregate (children in the order (0,0), (1,0), (0,1), (1,1))
+__global__ __launch_bounds__(BB) void k_mg_restrict(MgDev F, MgDev C, const double* __restrict__ r,
+                                                    const double* __restrict__ x, double* __restrict__ rc) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  if (c >= C.a * C.b) return;
+  const int I = c % C.a, J = c / C.a;
+  double s = 0.0;
+  for (int dj = 0; dj < 2; ++dj)
+    for (int di = 0; di < 2; ++di) {
+      const int i = F.o + 2 * I + di, j = F.o + 2 * J + dj;
+      if (F.active(i, j)) {
+        const int f = j * F.a + i;
+        s += r[f] - F.kx(x, i, j, f);
+      }
+    }
+  rc[c] = s;
+}
+
+// x += kAlpha xc of the cell's aggregate, on the unknowns
+__global__ __launch_bounds__(BB) void k_mg_prolong(MgDev F, MgDev C, const double* __restrict__ xc,
+                                                   double* __restrict__ x) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  if (c >= F.a * F.b) return;
+  const int i = c % F.a, j = c / F.a;
+  if (F.active(i, j)) x[c] += kAlpha * xc[((j - F.o) / 2) * C.a + (i - F.o) / 2];
+}
+
+// x = inv r over the level's unknowns (numbered row by row), 0 elsewhere; one block
+__global__ __launch_bounds__(BB) void k_mg_dense(MgDev L, int n, const double* __restrict__ inv,
+                                                 const double* __restrict__ r, double* __restrict__ x) {
+  const int wa = L.a - 2 * L.o;
+  for (int c = threadIdx.x; c < L.a * L.b; c += BB) {
+    const int i = c % L.a, j = c / L.a;
+    double s = 0.0;
+    if (L.active(i, j)) {
+      const int t = (j - L.o) * wa + (i - L.o);
+      for (int q = 0; q < n; ++q) {
+        const int cq = (q / wa + L.o) * L.a + (q % wa + L.o);
+        s += inv[(size_t)t * n + q] * r[cq];
+      }
+    }
+    x[c] = s;
+  }
+}
+
+// the block sums of r z over the box (when the last level is the box itself)
+__global__ __launch_bounds__(BB) void k_mg_dot(int n, const double* __restrict__ r, const double* __restrict__ z,
+                                               double* __restrict__ part) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  block_sums(c < n ? r[c] * z[c] : 0.0, 0.0, 0.0, part);
+}
+
+enum { R_INIT = 0, R_ALPHA, R_BETA, R_CHECK, R_RZ };
 
 // One block: the block partials summed in a fixed order, then the scalars of the CG recurrence.
 __global__ __launch_bounds__(BB) void k2_reduce(const double* __restrict__ part, int nblk, double* __restrict__ scal,
@@ -322,6 +409,8 @@ __global__ __launch_bounds__(BB) void k2_reduce(const double* __restrict__ part,
     const double rzn = sh[0][0], rz = scal[S_RZ];
     scal[S_BETA] = rz != 0.0 ? rzn / rz : 0.0;
     scal[S_RZ] = rzn;
+  } else if (mode == R_RZ) {
+    scal[S_RZ] = sh[0][0];
   } else {
     scal[S_RR] = sh[0][0];
   }
@@ -395,6 +484,69 @@ __global__ __launch_bounds__(BB) void k_bot_correct(BDims g, const double* __res
     if (e_ != hipSuccess) return e_;       \
   } while (0)
 
+MgDev mgdev(const BotState::MgLevel& L) { return MgDev{L.a, L.b, L.o, L.k}; }
+
+// z = V-cycle(r) on the box, and the block sums of r z into b.part
+hipError_t launch_vcycle(hipStream_t st, BotState& b) {
+  const int L = (int)b.mg.size() - 1;
+  std::vector<double*> xf(b.mg.size());  // each level's iterate after its pre-smoothing, then after its post-smoothing
+  std::vector<double*> xo(b.mg.size());  // the level's other buffer
+  for (int l = 0; l < L; ++l) {
+    const BotState::MgLevel& lv = b.mg[l];
+    const MgDev D = mgdev(lv);
+    const double* r = l == 0 ? b.r : lv.r;
+    double *xa = lv.x, *xb = lv.y;
+    const unsigned nb = nblocks((long long)lv.a * lv.b);
+    hipLaunchKernelGGL(k_mg_jac0, dim3(nb), dim3(BB), 0, st, D, r, xa);
+    BCHK(hipGetLastError());
+    for (int s = 1; s < kNu; ++s) {
+      hipLaunchKernelGGL(k_mg_jac, dim3(nb), dim3(BB), 0, st, D, r, (const double*)xa, xb, b.part, 0);
+      BCHK(hipGetLastError());
+      std::swap(xa, xb);
+    }
+    const BotState::MgLevel& cv = b.mg[l + 1];
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((long long)cv.a * cv.b)), dim3(BB), 0, st, D, mgdev(cv), r,
+                       (const double*)xa, cv.r);
+    BCHK(hipGetLastError());
+    xf[l] = xa;
+    xo[l] = xb;
+  }
+  {
+    const BotState::MgLevel& lv = b.mg[L];
+    double* x = L == 0 ? b.z : lv.x;
+    hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(BB), 0, st, mgdev(lv), b.mg_n, (const double*)b.mg_inv,
+                       (const double*)(L == 0 ? b.r : lv.r), x);
+    BCHK(hipGetLastError());
+    xf[L] = x;
+    if (L == 0) {
+      const int n = b.nx * b.ny;
+      hipLaunchKernelGGL(k_mg_dot, dim3(nblocks(n)), dim3(BB), 0, st, n, (const double*)b.r, (const double*)b.z,
+                         b.part);
+      return hipGetLastError();
+    }
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    const BotState::MgLevel& lv = b.mg[l];
+    const MgDev D = mgdev(lv);
+    const double* r = l == 0 ? b.r : lv.r;
+    const unsigned nb = nblocks((long long)lv.a * lv.b);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), (const double*)xf[l + 1],
+                       xf[l]);
+    BCHK(hipGetLastError());
+    double *xa = xf[l], *xb = xo[l];
+    for (int s = 1; s <= kNu; ++s) {
+      const bool last = l == 0 && s == kNu;
+      double* out = last ? b.z : xb;
+      hipLaunchKernelGGL(k_mg_jac, dim3(nb), dim3(BB), 0, st, D, r, (const double*)xa, out, b.part, last ? 1 : 0);
+      BCHK(hipGetLastError());
+      xb = xa;
+      xa = out;
+    }
+    xf[l] = xa;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_chunk(hipStream_t st, BotState& b) {
   const BDims g = dims(b);
   const int n = b.nx * b.ny;
@@ -406,12 +558,142 @@ hipError_t launch_chunk(hipStream_t st, BotState& b) {
     BCHK(hipGetLastError());
     hipLaunchKernelGGL(k2_reduce, dim3(1), dim3(BB), 0, st, (const double*)b.part, b.nblk, b.scal, (int)R_ALPHA);
     BCHK(hipGetLastError());
-    hipLaunchKernelGGL(k2_update, dim3(nblocks(n)), dim3(BB), 0, st, g, (const double*)b.coef, (const double*)b.S,
-                       b.x, (const double*)dnew, b.r, (const double*)b.q, b.z, b.part, (const double*)b.scal);
+    hipLaunchKernelGGL(k2_update, dim3(nblocks(n)), dim3(BB), 0, st, g, b.x, (const double*)dnew, b.r,
+                       (const double*)b.q, (const double*)b.scal);
     BCHK(hipGetLastError());
+    BCHK(launch_vcycle(st, b));
     hipLaunchKernelGGL(k2_reduce, dim3(1), dim3(BB), 0, st, (const double*)b.part, b.nblk, b.scal, (int)R_BETA);
     BCHK(hipGetLastError());
   }
+  return hipSuccess;
+}
+
+// The multigrid hierarchy from the box operator K = -S A2 on the inner rows (host, once per handle):
+// level 0 is the box with its edge excluded; a level's 2 x 2 aggregates of unknowns become the next
+// level's cells, with K_c = P^T K P for the piecewise-constant P (5-point again: a coupling inside an
+// aggregate adds to the coarse centre, one across its west/east/north/south side to that coarse
+// coupling), until at most kMgDense unknowns remain; that operator is inverted (Gauss-Jordan with
+// partial pivoting). Every loop runs in a fixed order, so the hierarchy is the same bits every time.
+hipError_t mg_build(hipStream_t st, BotState& b) {
+  const int nx = b.nx, ny = b.ny;
+  const size_t n = (size_t)nx * ny;
+  std::vector<double> coef(5 * n), S(n);
+  BCHK(hipMemcpyAsync(coef.data(), b.coef, 5 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  BCHK(hipMemcpyAsync(S.data(), b.S, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  BCHK(hipStreamSynchronize(st));
+  struct HLevel {
+    int a, b, o;
+    std::vector<double> k;
+  };
+  std::vector<HLevel> lv;
+  {
+    HLevel L0{nx, ny, 1, std::vector<double>(5 * n, 0.0)};
+    auto act = [&](int i, int j) { return i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1; };
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t c = (size_t)j * nx + i;
+        if (!act(i, j)) {
+          L0.k[c] = 1.0;
+          continue;
+        }
+        L0.k[c] = -S[c] * coef[c];
+        if (act(i - 1, j)) L0.k[n + c] = -S[c] * coef[n + c];
+        if (act(i + 1, j)) L0.k[2 * n + c] = -S[c] * coef[2 * n + c];
+        if (act(i, j - 1)) L0.k[3 * n + c] = -S[c] * coef[3 * n + c];
+        if (act(i, j + 1)) L0.k[4 * n + c] = -S[c] * coef[4 * n + c];
+      }
+    lv.push_back(std::move(L0));
+  }
+  auto unknowns = [](const HLevel& L) { return (long long)(L.a - 2 * L.o) * (L.b - 2 * L.o); };
+  while (unknowns(lv.back()) > kMgDense) {
+    const HLevel& F = lv.back();
+    const int wa = F.a - 2 * F.o, wb = F.b - 2 * F.o, ca = (wa + 1) / 2, cb = (wb + 1) / 2;
+    const size_t fn = (size_t)F.a * F.b, cn = (size_t)ca * cb;
+    HLevel C{ca, cb, 0, std::vector<double>(5 * cn, 0.0)};
+    const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+    for (int j = F.o; j < F.b - F.o; ++j)
+      for (int i = F.o; i < F.a - F.o; ++i) {
+        const size_t f = (size_t)j * F.a + i;
+        const int I = (i - F.o) / 2, J = (j - F.o) / 2;
+        const size_t cc = (size_t)J * ca + I;
+        C.k[cc] += F.k[f];
+        for (int p = 1; p < 5; ++p) {
+          const int ni = i + di[p], nj = j + dj[p];
+          if (ni < F.o || nj < F.o || ni >= F.a - F.o || nj >= F.b - F.o) continue;
+          const double v = F.k[p * fn + f];
+          if ((ni - F.o) / 2 == I && (nj - F.o) / 2 == J)
+            C.k[cc] += v;
+          else
+            C.k[p * cn + cc] += v;
+        }
+      }
+    lv.push_back(std::move(C));
+  }
+  // the dense inverse of the last level over its unknowns, numbered row by row
+  const HLevel& T = lv.back();
+  const int wa = T.a - 2 * T.o, m = (int)unknowns(T);
+  std::vector<double> M((size_t)m * m, 0.0), inv((size_t)m * m, 0.0);
+  const size_t tn = (size_t)T.a * T.b;
+  const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+  for (int t = 0; t < m; ++t) {
+    const int i = t % wa + T.o, j = t / wa + T.o;
+    const size_t c = (size_t)j * T.a + i;
+    for (int p = 0; p < 5; ++p) {
+      const int ni = i + di[p], nj = j + dj[p];
+      if (ni < T.o || nj < T.o || ni >= T.a - T.o || nj >= T.b - T.o) continue;
+      M[(size_t)t * m + (size_t)(nj - T.o) * wa + (ni - T.o)] += T.k[p * tn + c];
+    }
+    inv[(size_t)t * m + t] = 1.0;
+  }
+  for (int col = 0; col < m; ++col) {
+    int piv = col;
+    for (int rr = col + 1; rr < m; ++rr)
+      if (std::fabs(M[(size_t)rr * m + col]) > std::fabs(M[(size_t)piv * m + col])) piv = rr;
+    if (!(std::fabs(M[(size_t)piv * m + col]) > 0.0)) return hipErrorInvalidValue;
+    if (piv != col)
+      for (int q = 0; q < m; ++q) {
+        std::swap(M[(size_t)piv * m + q], M[(size_t)col * m + q]);
+        std::swap(inv[(size_t)piv * m + q], inv[(size_t)col * m + q]);
+      }
+    const double d = M[(size_t)col * m + col];
+    for (int q = 0; q < m; ++q) {
+      M[(size_t)col * m + q] /= d;
+      inv[(size_t)col * m + q] /= d;
+    }
+    for (int rr = 0; rr < m; ++rr) {
+      if (rr == col) continue;
+      const double f = M[(size_t)rr * m + col];
+      if (f == 0.0) continue;
+      for (int q = 0; q < m; ++q) {
+        M[(size_t)rr * m + q] -= f * M[(size_t)col * m + q];
+        inv[(size_t)rr * m + q] -= f * inv[(size_t)col * m + q];
+      }
+    }
+  }
+  // device copies: level 0 uses the PCG's r and z plus two iterate buffers of the box
+  auto dev = [&](double*& p, size_t cnt, const double* src) -> hipError_t {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, cnt * sizeof(double));
+    if (e != hipSuccess) return e;
+    b.mg_allocs.push_back(q);
+    p = static_cast<double*>(q);
+    if (src) return hipMemcpy(p, src, cnt * sizeof(double), hipMemcpyHostToDevice);
+    return hipMemset(p, 0, cnt * sizeof(double));
+  };
+  b.mg.assign(lv.size(), BotState::MgLevel{});
+  for (size_t l = 0; l < lv.size(); ++l) {
+    BotState::MgLevel& d = b.mg[l];
+    d.a = lv[l].a;
+    d.b = lv[l].b;
+    d.o = lv[l].o;
+    const size_t cnt = (size_t)d.a * d.b;
+    BCHK(dev(d.k, 5 * cnt, lv[l].k.data()));
+    BCHK(dev(d.x, cnt, nullptr));
+    BCHK(dev(d.y, cnt, nullptr));
+    if (l > 0) BCHK(dev(d.r, cnt, nullptr));
+  }
+  BCHK(dev(b.mg_inv, (size_t)m * m, inv.data()));
+  b.mg_n = m;
   return hipSuccess;
 }
 
@@ -431,7 +713,9 @@ hipError_t bot_setup(hipStream_t st, BotState& b) {
   b.iters = -1;  // no solve yet: the first one starts from zero
   hipLaunchKernelGGL(k_bot_coef, dim3(nblocks(n)), dim3(BB), 0, st, g, b.dlxu, b.dlyv, b.dlxp, b.dlyp, b.coef, b.S);
   BCHK(hipGetLastError());
-  return hipStreamSynchronize(st);
+  BCHK(hipStreamSynchronize(st));
+  if (b.nx > 2 && b.ny > 2) BCHK(mg_build(st, b));
+  return hipSuccess;
 }
 
 hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const double* divg, double rtol, int maxit) {
@@ -450,15 +734,20 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
                      b.x, b.d0, b.d1, cold);
   BCHK(hipGetLastError());
   hipLaunchKernelGGL(k2_resid, dim3(nblocks(n)), dim3(BB), 0, st, g, (const double*)b.coef, (const double*)b.S,
-                     (const double*)b.b, (const double*)b.x, b.r, b.z, b.part, 1);
+                     (const double*)b.b, (const double*)b.x, b.r, b.part, 1);
   BCHK(hipGetLastError());
   hipLaunchKernelGGL(k2_reduce, dim3(1), dim3(BB), 0, st, (const double*)b.part, b.nblk, b.scal, (int)R_INIT);
   BCHK(hipGetLastError());
+  const bool has_inner = b.nx > 2 && b.ny > 2;
+  if (has_inner) {  // z = V-cycle(r) and r.z of the first iteration
+    BCHK(launch_vcycle(st, b));
+    hipLaunchKernelGGL(k2_reduce, dim3(1), dim3(BB), 0, st, (const double*)b.part, b.nblk, b.scal, (int)R_RZ);
+    BCHK(hipGetLastError());
+  }
   BCHK(read_scal(st, b));
   const double bb = b.scal_host[S_BB], tol2 = rtol * rtol * bb;
   double rr = b.scal_host[S_RR];
   int iters = 0, status = 0;
-  const bool has_inner = b.nx > 2 && b.ny > 2;
   while (has_inner && rr > tol2 && iters < maxit) {
     if (!b.chunk) {
       hipGraph_t graph = nullptr;
@@ -473,7 +762,7 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
     BCHK(hipGraphLaunch(b.chunk, st));
     iters += kBotChunk;
     hipLaunchKernelGGL(k2_resid, dim3(nblocks(n)), dim3(BB), 0, st, g, (const double*)b.coef, (const double*)b.S,
-                       (const double*)b.b, (const double*)b.x, b.r, b.z, b.part, 0);
+                       (const double*)b.b, (const double*)b.x, b.r, b.part, 0);
     BCHK(hipGetLastError());
     hipLaunchKernelGGL(k2_reduce, dim3(1), dim3(BB), 0, st, (const double*)b.part, b.nblk, b.scal, (int)R_CHECK);
     BCHK(hipGetLastError());
@@ -502,6 +791,10 @@ hipError_t bot_correct(hipStream_t st, const BotState& b, const OcvArrays& a, do
 }
 
 void bot_release(BotState& b) {
+  for (void* p : b.mg_allocs) hipFree(p);
+  b.mg_allocs.clear();
+  b.mg.clear();
+  b.mg_inv = nullptr;
   if (b.chunk) hipGraphExecDestroy(b.chunk);
   b.chunk = nullptr;
   if (b.ev0) hipEventDestroy(b.ev0);

@@ -1,12 +1,14 @@
 """GPU parity for the 2-D bottom-pressure solve (CG2D4NONHY: ocvtot_nh.f90:64-147, mo_petsc.f90:147-208,
 252-374, 401-538) run inside orctm_nh_divergence when options.bottom_solve is set, through the C ABI.
+The 2-D solve is CG preconditioned by a multigrid V-cycle, restated in the oracle (orc_solve2d).
 
 Bar:
 * DIVG_bot_surf bit-exact against the oracle (same operations in the same order), also on a second step
   where the module arrays BOTGU/BOTGV/BOTGW carry over from the first;
 * PNH_bot within 1e-8 (relative, 2-norm) of SciPy's direct solve of compute_matrix_2d's matrix with the
   oracle's right-hand side, its residual recomputed with the oracle's operator <= 1e-10 relative, the
-  box-edge rows exactly b / A_cc, the halo as bounds_exch leaves it;
+  box-edge rows exactly b / A_cc, the halo as bounds_exch leaves it, and the iterations within one
+  residual check (8) of the oracle's V-cycle PCG;
 * the corrected DIVG bit-exact against the oracle's corrections fed the device's PNH_bot, and the 3-D
   solve from it bit-identical to a solve from that DIVG handed over by the host;
 * the warm start: the same step again needs no 2-D iterations.
@@ -67,6 +69,9 @@ def test_bottom_rhs_solve_and_correction(name):
     assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
     rr = np.linalg.norm(b - oracle.op2d_apply(g, c, x)) / np.linalg.norm(b)
     assert rr <= 1e-10 and out["rel_res"] <= 1e-10
+    # the oracle's V-cycle PCG from zero (the first solve is cold) stops at the same residual check
+    _, it_cpu, _ = oracle.solve2d(g, c, b, rtol=1e-10)
+    assert abs(out["iterations"] - it_cpu) <= 8, (out["iterations"], it_cpu)
     nx, ny = g.nx, g.ny
     i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
     edge = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1)
