@@ -40,6 +40,8 @@ struct Group {
   int h = 12;
   int ntiles = 0;
   Tile* d_tiles = nullptr;
+  int npairs = 0;
+  Tile* d_pairs = nullptr;  // the tiles with an even bx, for the column update over tile pairs
 };
 
 struct Ctx {
@@ -253,8 +255,10 @@ void ctx_destroy(Ctx& c) {
   for (auto& kv : c.graphs) hipGraphExecDestroy(kv.second);
   if (c.d_sys) hipFree(c.d_sys);
   if (c.d_tiles) hipFree(c.d_tiles);
-  for (Group& g : c.groups)
+  for (Group& g : c.groups) {
     if (g.d_tiles) hipFree(g.d_tiles);
+    if (g.d_pairs) hipFree(g.d_pairs);
+  }
   if (c.d_outer) hipFree(c.d_outer);
   if (c.d_inner) hipFree(c.d_inner);
   if (c.h_outer) hipHostFree(c.h_outer);
@@ -277,7 +281,8 @@ hipError_t iter_bwd(Ctx& c, int raw, hipEvent_t e0 = nullptr, hipEvent_t e1 = nu
 
 hipError_t iter_fwd(Ctx& c, bool init, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (e0) HIPCHK(hipEventRecord(e0, c.stream));
-  for (const Group& g : c.groups) HIPCHK(launch_fwd(c.stream, c.fp32, init, g.h, c.d_sys, g.d_tiles, g.ntiles));
+  for (const Group& g : c.groups)
+    HIPCHK(launch_fwd(c.stream, c.fp32, init, g.h, c.d_sys, g.d_tiles, g.ntiles, g.d_pairs, g.npairs));
   if (e1) HIPCHK(hipEventRecord(e1, c.stream));
   return launch_inner_reduce(c.stream, c.fp32, init ? PH_INIT : PH_FWD, c.d_sys, c.nsys);
 }
@@ -302,6 +307,15 @@ int size_class(const Sys& s) { return ceil_log2((long long)s.nx * s.ny); }
 
 void order_into(std::vector<Tile>& nat, const Sys& s, int q, int h) {
   for (const Tile& t : system_tiles(q, s.nx, s.ny, h)) nat.push_back(t);
+}
+
+// The tiles with an even bx, in the same order: one block of the paired column update runs each with
+// its right neighbour (bx + 1, when the system has one).
+std::vector<Tile> pair_tiles(const std::vector<Tile>& nat) {
+  std::vector<Tile> out;
+  for (const Tile& t : nat)
+    if (t.bx % 2 == 0) out.push_back(t);
+  return out;
 }
 
 // Scratch copies of some systems of a context, for timing the inner kernels under a tile-height
@@ -345,22 +359,27 @@ struct TuneSet {
   }
   // one warm round, then 3 timed rounds of (matvec, column update), heights h[q] per system
   hipError_t time(const std::vector<int>& h, float* ms) {
-    std::vector<Tile*> d_tiles;
-    std::vector<int> nts, hs;
+    std::vector<Tile*> d_tiles, d_pairs;
+    std::vector<int> nts, nps, hs;
     hipError_t e = hipSuccess;
     for (int hc : kTileHeights) {
       std::vector<Tile> nat;
       for (size_t q = 0; q < ts.size(); ++q)
         if (h[q] == hc) order_into(nat, ts[q], (int)q, hc);
       if (nat.empty()) continue;
-      std::vector<Tile> ord = xcd_order(nat);
-      Tile* d = nullptr;
+      std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
+      Tile *d = nullptr, *dp = nullptr;
       e = hipMalloc(&d, sizeof(Tile) * ord.size());
       if (e != hipSuccess) break;
       d_tiles.push_back(d);
+      e = hipMalloc(&dp, sizeof(Tile) * prs.size());
+      if (e != hipSuccess) break;
+      d_pairs.push_back(dp);
       nts.push_back((int)ord.size());
+      nps.push_back((int)prs.size());
       hs.push_back(hc);
       e = hipMemcpy(d, ord.data(), sizeof(Tile) * ord.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(dp, prs.data(), sizeof(Tile) * prs.size(), hipMemcpyHostToDevice);
       if (e != hipSuccess) break;
     }
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
@@ -368,12 +387,13 @@ struct TuneSet {
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
         e = launch_bwd(c.stream, c.fp32, hs[g], d_ts, d_tiles[g], nts[g], 0);
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
-        e = launch_fwd(c.stream, c.fp32, false, hs[g], d_ts, d_tiles[g], nts[g]);
+        e = launch_fwd(c.stream, c.fp32, false, hs[g], d_ts, d_tiles[g], nts[g], d_pairs[g], nps[g]);
     }
     if (e == hipSuccess) e = hipEventRecord(c.ev1, c.stream);
     if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, c.ev0, c.ev1);
     for (Tile* d : d_tiles) hipFree(d);
+    for (Tile* d : d_pairs) hipFree(d);
     return e;
   }
 };
@@ -474,13 +494,16 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
     for (int q = 0; q < c.nsys; ++q)
       if (c.heights[q] == hc) order_into(nat, dev[q], q, hc);
     if (nat.empty()) continue;
-    std::vector<Tile> ord = xcd_order(nat);
+    std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
     Group g;
     g.h = hc;
     g.ntiles = (int)ord.size();
+    g.npairs = (int)prs.size();
     HIPCHK(hipMalloc(&g.d_tiles, sizeof(Tile) * g.ntiles));
     c.groups.push_back(g);
+    HIPCHK(hipMalloc(&c.groups.back().d_pairs, sizeof(Tile) * g.npairs));
     HIPCHK(hipMemcpy(g.d_tiles, ord.data(), sizeof(Tile) * g.ntiles, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.groups.back().d_pairs, prs.data(), sizeof(Tile) * g.npairs, hipMemcpyHostToDevice));
   }
   return hipSuccess;
 }

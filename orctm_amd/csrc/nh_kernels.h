@@ -41,7 +41,10 @@ hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Ti
 // groups of the phase have been launched. init: x = 0 and the first forward elimination.
 constexpr int kTileHeights[3] = {4, 8, 12};
 enum InnerPhase { PH_INIT = 0, PH_BWD = 1, PH_FWD = 2 };
-hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+// d_pairs: the tiles of d_tiles with an even bx (one block runs that tile and the next one in x), for
+// launches of kDeepTiles tiles or more.
+hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                      const Tile* d_pairs, int npairs);
 // raw = 1 (verification only): q = C w without the back substitution.
 hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw);
 hipError_t launch_inner_reduce(hipStream_t st, bool fp32, InnerPhase ph, const Sys* d_sys, int nsys);

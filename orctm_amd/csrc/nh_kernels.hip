@@ -912,6 +912,192 @@ __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems,
   publish_rows<T, H, H>(s, t.bx, t.by, rho, rr);
 }
 
+// Two adjacent columns per lane: vectors of two values in one load or store.
+template <typename T>
+struct V2;
+template <>
+struct V2<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct V2<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+// the two factors of adjacent columns: a pair of fp16 in one 32-bit word (FP32), a vector of 2 (FP64)
+template <typename T>
+struct Fac2 {
+  using S = typename V2<T>::type;
+  __device__ static void ld_nt(gptr<const T> p, size_t i, T& a, T& b) {
+    const S v = __builtin_nontemporal_load(reinterpret_cast<gptr<const S>>(p + i));
+    a = v.x;
+    b = v.y;
+  }
+};
+template <>
+struct Fac2<float> {
+  __device__ static void ld_nt(gptr<const uint16_t> p, size_t i, float& a, float& b) {
+    const uint32_t v = __builtin_nontemporal_load(reinterpret_cast<gptr<const uint32_t>>(p + i));
+    a = half_val((uint16_t)(v & 0xffffu));
+    b = half_val((uint16_t)(v >> 16));
+  }
+};
+
+// The sum over a tile row held by a half-wavefront, lane h holding columns 2h (v0) and 2h+1 (v1):
+// the same tree as wave_sum over one column per lane (column c paired with c+32, c+16, ..., c+1), so
+// the row sums are the same bits in either layout.
+template <typename T>
+__device__ __forceinline__ T half_row_sum(T v0, T v1) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
+  return v0 + v1;
+}
+
+// publish_rows for a block of two adjacent tiles (bx, bx+1) whose wavefront w holds row j0 + w of
+// both, the first tile in lanes 0-31 and the second in lanes 32-63, with their row sums already formed
+// (half_row_sum): the same slots and the same group sums per tile.
+template <typename T, int H>
+__device__ __forceinline__ void publish_pair_rows(const Sys& s, int bx, int by, T a, T b) {
+  static_assert(H % RG == 0, "row groups must not straddle tiles");
+  __shared__ T sh[2][2 * H];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, half = l >> 5;
+  __syncthreads();
+  if ((l & 31) == 0) {
+    sh[half][w] = a;
+    sh[half][H + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * (H / RG)) {
+    const int half2 = threadIdx.x / (H / RG), g = threadIdx.x % (H / RG), grow = by * (H / RG) + g;
+    const int tx = bx + half2;
+    if (grow * RG < s.ny && tx < s.tiles_x) {
+      const T* v = sh[half2];
+      const T sa = (v[4 * g] + v[4 * g + 1]) + (v[4 * g + 2] + v[4 * g + 3]);
+      const T sb = (v[H + 4 * g] + v[H + 4 * g + 1]) + (v[H + 4 * g + 2] + v[H + 4 * g + 3]);
+      gptr<T> part = G<T>(s.partials);
+      const size_t slot = (size_t)grow * s.tiles_x + tx;
+      part[2 * slot] = sa;
+      part[2 * slot + 1] = sb;
+    }
+  }
+}
+
+// k_fwd over a pair of adjacent 64 x H tiles (bx even, bx+1): wavefront w holds row j0 + w of both,
+// lane l the columns 2l and 2l+1 of the pair (the first tile in lanes 0-31), so every load and store
+// is one contiguous 512-byte run per wavefront. The same operations per column, in the same order, and
+// the same row sums per tile (half_row_sum), so the same bits as k_fwd; half as many, twice as wide
+// memory operations stream the column march faster (1024x768x60: 6.0 against 5.0-5.5 TB/s for a march
+// of this shape).
+template <typename T, bool INIT, int H, int U>
+__global__ __launch_bounds__(64 * H) void k_fwd2(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
+  using T2 = typename V2<T>::type;
+  const Tile t = tiles[blockIdx.x];
+  const Sys& s = systems[t.sys];
+  Outer* o = s.outer;
+  Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
+  if (INIT) {
+    if (o->odone != RUN) return;
+  } else {
+    if (in->done != RUN) return;
+  }
+  const int i = t.bx * TX + 2 * (threadIdx.x & 63), j = t.by * H + (threadIdx.x >> 6);
+  const size_t P = s.plane, col = (size_t)j * s.pitch + i;
+  const int nz = s.nz;
+  const bool ok0 = i < s.nx && j < s.ny, ok1 = i + 1 < s.nx && j < s.ny;
+  gptr<T2> x = reinterpret_cast<gptr<T2>>(G<T>(s.x));
+  gptr<T2> r = reinterpret_cast<gptr<T2>>(G<T>(s.r));
+  gptr<T2> w = reinterpret_cast<gptr<T2>>(G<T>(s.z));
+  gptr<T> x1 = G<T>(s.x), r1 = G<T>(s.r), w1 = G<T>(s.z);
+  gptr<const T2> q = reinterpret_cast<gptr<const T2>>(G<const T>(s.q));
+  gptr<const typename Fac<T>::S> L = G<const typename Fac<T>::S>(s.L);
+  gptr<const typename Fac<T>::S> invD = G<const typename Fac<T>::S>(s.invD);
+  const T alpha = INIT ? T(0) : in->alpha;
+  T rho0 = T(0), rr0 = T(0), rho1 = T(0), rr1 = T(0);
+  if (ok0) {
+    // the pair's element index is col / 2 (i is even and the pitch a multiple of 32); with i + 1
+    // beyond nx the second column is padding, read but never stored or summed
+    const size_t e2 = col / 2, P2 = P / 2;
+    T cs0 = T(1), cs1 = T(1);
+    if constexpr (std::is_same<T, float>::value) {
+      const T2 c = __builtin_nontemporal_load(reinterpret_cast<gptr<const T2>>(G<const float>(s.dscale)) + e2);
+      cs0 = c.x;
+      cs1 = c.y;
+    }
+    struct Batch {
+      T2 r[U], q[U];
+      T l0[U], l1[U], d0[U], d1[U];
+    };
+    Batch b0, b1;
+    auto load = [&](int k0, Batch& b) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u < nz ? k0 + u : nz - 1;
+        const size_t idx = col + (size_t)k * P;
+        b.r[u] = __builtin_nontemporal_load(r + (e2 + (size_t)k * P2));
+        Fac2<T>::ld_nt(L, idx, b.l0[u], b.l1[u]);
+        Fac2<T>::ld_nt(invD, idx, b.d0[u], b.d1[u]);
+        b.d0[u] *= cs0;
+        b.d1[u] *= cs1;
+        if (!INIT) b.q[u] = __builtin_nontemporal_load(q + (e2 + (size_t)k * P2));
+      }
+    };
+    T y0 = T(0), y1 = T(0);
+    auto compute = [&](int k0, const Batch& b) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (k0 + u < nz) {
+          const size_t e = e2 + (size_t)(k0 + u) * P2, idx = col + (size_t)(k0 + u) * P;
+          T rk0 = b.r[u].x, rk1 = b.r[u].y;
+          if (INIT) {
+            if (ok1) {
+              __builtin_nontemporal_store(T2{T(0), T(0)}, x + e);
+            } else {
+              st_nt<T>(x1, idx, T(0));
+            }
+          } else {
+            rk0 = fmad(-alpha, b.q[u].x, rk0);
+            rk1 = fmad(-alpha, b.q[u].y, rk1);
+            if (ok1) {
+              __builtin_nontemporal_store(T2{rk0, rk1}, r + e);
+            } else {
+              st_nt<T>(r1, idx, rk0);
+            }
+          }
+          rr0 = fmad(rk0, rk0, rr0);
+          rr1 = fmad(rk1, rk1, rr1);
+          const T ya = fmad(-b.l0[u], y0, rk0), yb = fmad(-b.l1[u], y1, rk1);
+          const T wa = ya * b.d0[u], wb = yb * b.d1[u];
+          rho0 = fmad(ya, wa, rho0);
+          rho1 = fmad(yb, wb, rho1);
+          if (ok1) {
+            __builtin_nontemporal_store(T2{wa, wb}, w + e);
+          } else {
+            st_nt<T>(w1, idx, wa);
+          }
+          y0 = ya;
+          y1 = yb;
+        }
+      }
+    };
+    load(0, b0);
+    for (int k0 = 0; k0 < nz; k0 += 2 * U) {
+      load(k0 + U, b1);
+      compute(k0, b0);
+      if (k0 + U >= nz) break;
+      load(k0 + 2 * U, b0);
+      compute(k0 + U, b1);
+    }
+    if (!ok1) {
+      rho1 = T(0);
+      rr1 = T(0);
+    }
+  }
+  const T rho = half_row_sum(rho0, rho1), rr = half_row_sum(rr0, rr1);
+  publish_pair_rows<T, H>(s, t.bx, t.by, rho, rr);
+}
+
 // Threads of the matvec kernel (tile of 64 x H columns): H wavefronts own the tile rows (one column
 // each, centre slots); three more wavefronts own the halo: the north row, the south row and the 2*H
 // west/east cells. Every thread runs the back-substitution recurrence of exactly one column, so no
@@ -1486,6 +1672,16 @@ hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile
 }
 
 template <typename T, int H, int U>
+static hipError_t fwd_hu2(hipStream_t st, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+  const dim3 grid(ntiles), block(64 * H);
+  if (init)
+    hipLaunchKernelGGL((k_fwd2<T, true, H, U>), grid, block, 0, st, d_sys, d_tiles);
+  else
+    hipLaunchKernelGGL((k_fwd2<T, false, H, U>), grid, block, 0, st, d_sys, d_tiles);
+  return hipGetLastError();
+}
+
+template <typename T, int H, int U>
 static hipError_t fwd_hu(hipStream_t st, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
   const dim3 grid(ntiles), block(64 * H);
   if (init)
@@ -1499,14 +1695,17 @@ static hipError_t fwd_hu(hipStream_t st, bool init, const Sys* d_sys, const Tile
 // CUs is a few column chains, and longer batches keep each chain's loads a memory round trip ahead of
 // their use: 16 levels (FP32, 4-row tiles) or 8, as many as fit the registers without spilling.
 template <int H>
-static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                        const Tile* d_pairs, int npairs) {
   constexpr int UF = H < 8 ? 16 : 8, UD = H < 12 ? 8 : 4;
   const bool deep = ntiles < kDeepTiles;
+  // load batches of the paired form: as many levels as fit the registers without spilling
+  constexpr int PF = 2, PD = H < 12 ? 4 : 2;
   if (fp32)
     return deep ? fwd_hu<float, H, UF>(st, init, d_sys, d_tiles, ntiles)
-                : fwd_hu<float, H, 4>(st, init, d_sys, d_tiles, ntiles);
+                : fwd_hu2<float, H, PF>(st, init, d_sys, d_pairs, npairs);
   return deep ? fwd_hu<double, H, UD>(st, init, d_sys, d_tiles, ntiles)
-              : fwd_hu<double, H, 4>(st, init, d_sys, d_tiles, ntiles);
+              : fwd_hu2<double, H, PD>(st, init, d_sys, d_pairs, npairs);
 }
 
 template <typename T, int H, int D>
@@ -1540,12 +1739,13 @@ static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile*
     return bwd_stream<double, H, 1>(st, d_sys, d_tiles, ntiles, raw);
 }
 
-hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles) {
+hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                      const Tile* d_pairs, int npairs) {
   if (ntiles <= 0) return hipSuccess;
   switch (h) {
-    case 4: return fwd_h<4>(st, fp32, init, d_sys, d_tiles, ntiles);
-    case 8: return fwd_h<8>(st, fp32, init, d_sys, d_tiles, ntiles);
-    case 12: return fwd_h<12>(st, fp32, init, d_sys, d_tiles, ntiles);
+    case 4: return fwd_h<4>(st, fp32, init, d_sys, d_tiles, ntiles, d_pairs, npairs);
+    case 8: return fwd_h<8>(st, fp32, init, d_sys, d_tiles, ntiles, d_pairs, npairs);
+    case 12: return fwd_h<12>(st, fp32, init, d_sys, d_tiles, ntiles, d_pairs, npairs);
     default: return hipErrorInvalidValue;
   }
 }
