@@ -624,7 +624,8 @@ def bench_bottom(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_iteration": mg_iteration_bytes(g.nx, g.ny),
                      "note": "the PCG's 128 B per box cell plus 288 B per cell of every multigrid level; about "
-                             "45 launches per iteration on grids down to 8 x 6 cells make the solve launch-bound, "
+                             "36 launches per iteration on grids down to 64 x 48 cells (one block below that) make "
+                             "the solve launch-bound, "
                              "and its working set (about 130 MB) stays in the Infinity Cache"},
         "cpu_baseline": {"value": round((cpu_it / cpu_s) / per_solve, 4), "unit": "solves/s", "cores": 1,
                          "kind": "port",

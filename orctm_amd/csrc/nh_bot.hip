@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "../../include/orctm_nh.h"
@@ -364,6 +365,153 @@ __global__ __launch_bounds__(BB) void k_mg_dot(int n, const double* __restrict__
   block_sums(c < n ? r[c] * z[c] : 0.0, 0.0, 0.0, part);
 }
 
+// ---- the coarse tail of the V-cycle in one block ---------------------------------------------------
+// Below a few hundred cells a level's launches cost far more than its work. The last levels of the
+// hierarchy (each at most kTailCells cells, halo included: one cell per thread) therefore run in one
+// block: each thread keeps its cell's 5 operator entries in registers, the iterates, right-hand sides
+// and residuals of every tail level sit in LDS, and the phases the separate kernels would run are
+// separated by block barriers. Every cell's arithmetic is the separate kernels' (k_mg_jac0, k_mg_jac,
+// k_mg_restrict split into the cell residuals and the same 4-child sums in the same order,
+// k_mg_prolong, k_mg_dense), so the V-cycle gives the same bits.
+constexpr int kTailCells = 1024;
+constexpr int kTailLevels = 4;
+constexpr size_t kTailLds = 64 * 1024;  // LDS bytes the tail may use (4 doubles per cell)
+
+struct MgTail {
+  int n;  // levels in the tail; the last is the dense level
+  int a[kTailLevels], b[kTailLevels], o[kTailLevels];
+  const double* k[kTailLevels];
+  int dense_n;
+  const double* inv;
+  const double* r0;  // right-hand side of the first tail level (the restriction from the level above)
+  double* x0;        // the first tail level's result, where the prolongation from it reads
+};
+
+__global__ __launch_bounds__(kTailCells) void k_mg_tail(MgTail t) {
+  extern __shared__ double sm[];
+  const int c = threadIdx.x;
+  // LDS planes per level: X, Y (the iterate pair), R (right-hand side), E (cell residuals)
+  double *X[kTailLevels], *Y[kTailLevels], *R[kTailLevels], *E[kTailLevels];
+  double kr[kTailLevels][5];
+  {
+    double* p = sm;
+#pragma unroll
+    for (int l = 0; l < kTailLevels; ++l) {
+      const int n = l < t.n ? t.a[l] * t.b[l] : 0;
+      X[l] = p;
+      Y[l] = p + n;
+      R[l] = p + 2 * n;
+      E[l] = p + 3 * n;
+      p += 4 * n;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) kr[l][q] = (c < n) ? t.k[l][(size_t)q * n + c] : 0.0;
+    }
+  }
+  if (c < t.a[0] * t.b[0]) R[0][c] = t.r0[c];
+  __syncthreads();
+  auto active = [&](int l, int i, int j) {
+    return i >= t.o[l] && j >= t.o[l] && i < t.a[l] - t.o[l] && j < t.b[l] - t.o[l];
+  };
+  // K x at this thread's cell of level l (MgDev::kx with the entries from registers)
+  auto kx = [&](int l, const double* x, int i, int j) {
+    const int a = t.a[l];
+    double s = kr[l][0] * x[c];
+    if (i > 0) s += kr[l][1] * x[c - 1];
+    if (i < a - 1) s += kr[l][2] * x[c + 1];
+    if (j > 0) s += kr[l][3] * x[c - a];
+    if (j < t.b[l] - 1) s += kr[l][4] * x[c + a];
+    return s;
+  };
+  auto jac = [&](int l, const double* x, double* y) {
+    if (c < t.a[l] * t.b[l]) {
+      const int i = c % t.a[l], j = c / t.a[l];
+      double v = 0.0;
+      if (active(l, i, j)) v = x[c] + kOmega * (R[l][c] - kx(l, x, i, j)) / kr[l][0];
+      y[c] = v;
+    }
+    __syncthreads();
+  };
+  double* xf[kTailLevels];
+  double* xo[kTailLevels];
+  const int L = t.n - 1;
+#pragma unroll
+  for (int l = 0; l < kTailLevels - 1; ++l) {
+    if (l >= L) break;
+    const int a = t.a[l], n = a * t.b[l];
+    double *xa = X[l], *xb = Y[l];
+    if (c < n) {
+      const int i = c % a, j = c / a;
+      xa[c] = active(l, i, j) ? kOmega * R[l][c] / kr[l][0] : 0.0;
+    }
+    __syncthreads();
+    for (int s = 1; s < kNu; ++s) {
+      jac(l, xa, xb);
+      double* tmp = xa;
+      xa = xb;
+      xb = tmp;
+    }
+    if (c < n) {
+      const int i = c % a, j = c / a;
+      if (active(l, i, j)) E[l][c] = R[l][c] - kx(l, xa, i, j);
+    }
+    __syncthreads();
+    const int ca = t.a[l + 1];
+    if (c < ca * t.b[l + 1]) {
+      const int I = c % ca, J = c / ca;
+      double s = 0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int i = t.o[l] + 2 * I + di, j = t.o[l] + 2 * J + dj;
+          if (active(l, i, j)) s += E[l][j * a + i];
+        }
+      R[l + 1][c] = s;
+    }
+    __syncthreads();
+    xf[l] = xa;
+    xo[l] = xb;
+  }
+  // the dense level: x = inv r over its unknowns (numbered row by row), 0 elsewhere
+#pragma unroll
+  for (int l = 0; l < kTailLevels; ++l) {
+    if (l != L) continue;
+    double* x = X[l];
+    const int a = t.a[l], o = t.o[l], wa = a - 2 * o;
+    if (c < a * t.b[l]) {
+      const int i = c % a, j = c / a;
+      double s = 0.0;
+      if (active(l, i, j)) {
+        const int r = (j - o) * wa + (i - o);
+        for (int q = 0; q < t.dense_n; ++q) {
+          const int cq = (q / wa + o) * a + (q % wa + o);
+          s += t.inv[(size_t)r * t.dense_n + q] * R[l][cq];
+        }
+      }
+      x[c] = s;
+    }
+    __syncthreads();
+    xf[l] = x;
+  }
+#pragma unroll
+  for (int l = kTailLevels - 2; l >= 0; --l) {
+    if (l >= L) continue;
+    const int a = t.a[l], n = a * t.b[l], ca = t.a[l + 1];
+    if (c < n) {
+      const int i = c % a, j = c / a;
+      if (active(l, i, j)) xf[l][c] += kAlpha * xf[l + 1][((j - t.o[l]) / 2) * ca + (i - t.o[l]) / 2];
+    }
+    __syncthreads();
+    double *xa = xf[l], *xb = xo[l];
+    for (int s = 1; s <= kNu; ++s) {
+      double* out = xb;
+      jac(l, xa, out);
+      xb = xa;
+      xa = out;
+    }
+    xf[l] = xa;
+  }
+  if (c < t.a[0] * t.b[0]) t.x0[c] = xf[0][c];
+}
+
 enum { R_INIT = 0, R_ALPHA, R_BETA, R_CHECK, R_RZ };
 
 // One block: the block partials summed in a fixed order, then the scalars of the CG recurrence.
@@ -487,11 +635,30 @@ __global__ __launch_bounds__(BB) void k_bot_correct(BDims g, const double* __res
 MgDev mgdev(const BotState::MgLevel& L) { return MgDev{L.a, L.b, L.o, L.k}; }
 
 // z = V-cycle(r) on the box, and the block sums of r z into b.part
+// The first level of the one-block tail: the coarsest levels (not the box) that each have at most
+// kTailCells cells, at most kTailLevels of them and kTailLds bytes of LDS together. The level count
+// when there is no tail (the box is the last level).
+static int tail_first(const BotState& b) {
+  const int L = (int)b.mg.size() - 1;
+  int T = L + 1;
+  size_t bytes = 0;
+  while (T - 1 >= 1) {
+    const BotState::MgLevel& lv = b.mg[T - 1];
+    const size_t n = (size_t)lv.a * lv.b, need = 4 * sizeof(double) * n;
+    if (n > (size_t)kTailCells || L - T + 2 > kTailLevels || bytes + need > kTailLds) break;
+    bytes += need;
+    --T;
+  }
+  return T;
+}
+
 hipError_t launch_vcycle(hipStream_t st, BotState& b) {
   const int L = (int)b.mg.size() - 1;
+  const int T = tail_first(b);  // levels T..L run in the one-block tail
+  const int sep = T <= L ? T : L;                    // levels 0..sep-1 have launches of their own
   std::vector<double*> xf(b.mg.size());  // each level's iterate after its pre-smoothing, then after its post-smoothing
   std::vector<double*> xo(b.mg.size());  // the level's other buffer
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < sep; ++l) {
     const BotState::MgLevel& lv = b.mg[l];
     const MgDev D = mgdev(lv);
     const double* r = l == 0 ? b.r : lv.r;
@@ -511,7 +678,38 @@ hipError_t launch_vcycle(hipStream_t st, BotState& b) {
     xf[l] = xa;
     xo[l] = xb;
   }
-  {
+  if (T <= L) {
+    MgTail t{};
+    t.n = L - T + 1;
+    size_t lds = 0;
+    for (int q = 0; q < t.n; ++q) {
+      const BotState::MgLevel& lv = b.mg[T + q];
+      t.a[q] = lv.a;
+      t.b[q] = lv.b;
+      t.o[q] = lv.o;
+      t.k[q] = lv.k;
+      lds += 4 * sizeof(double) * (size_t)lv.a * lv.b;
+    }
+    t.dense_n = b.mg_n;
+    t.inv = b.mg_inv;
+    t.r0 = b.mg[T].r;
+    // the buffer in which the per-level kernels leave level T's result (their ping-pong, replayed)
+    double* x0 = b.mg[T].x;
+    if (T < L) {
+      double *xa = b.mg[T].x, *xb = b.mg[T].y;
+      for (int s = 1; s < kNu; ++s) std::swap(xa, xb);
+      for (int s = 1; s <= kNu; ++s) {
+        double* out = xb;
+        xb = xa;
+        xa = out;
+      }
+      x0 = xa;
+    }
+    t.x0 = x0;
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailCells), lds, st, t);
+    BCHK(hipGetLastError());
+    xf[T] = x0;
+  } else {
     const BotState::MgLevel& lv = b.mg[L];
     double* x = L == 0 ? b.z : lv.x;
     hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(BB), 0, st, mgdev(lv), b.mg_n, (const double*)b.mg_inv,
@@ -525,7 +723,7 @@ hipError_t launch_vcycle(hipStream_t st, BotState& b) {
       return hipGetLastError();
     }
   }
-  for (int l = L - 1; l >= 0; --l) {
+  for (int l = sep - 1; l >= 0; --l) {
     const BotState::MgLevel& lv = b.mg[l];
     const MgDev D = mgdev(lv);
     const double* r = l == 0 ? b.r : lv.r;
