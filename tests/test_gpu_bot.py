@@ -92,6 +92,40 @@ def test_bottom_rhs_solve_and_correction(name):
     assert bits_equal(p_dev, p2)
 
 
+# Box sizes (IE-2) x (JE-2) for the V-cycle's one-block tail (its coarsest levels, each of at most 1024
+# cells, at most 4 of them and 64 KB of LDS, run in one block), with the tail each one gives under the
+# coarsening of mg_build (the box, then the unknown extents halved, down to at most 64 unknowns).
+TAIL_BOXES = {
+    (8, 8): "no level below the box (the box itself is solved densely, no tail)",
+    (16, 16): "levels 16x16, 7x7: the tail is the dense level alone",
+    (30, 30): "levels 30x30, 14x14, 7x7: a tail of 2 levels",
+    (58, 58): "levels 58x58 .. 7x7: a tail of 3 levels from level 1",
+    (1026, 4): "levels 1026x4, 512x1 .. 64x1: a tail of 4 one-row levels",
+    (200, 150): "levels 200x150 .. 7x5: a tail of 3 levels from level 3",
+}
+
+
+@pytest.mark.parametrize("box", list(TAIL_BOXES))
+def test_bottom_solve_over_multigrid_tail_shapes(box):
+    nx, ny = box
+    print(f"[bot] box {nx}x{ny}: {TAIL_BOXES[box]}", flush=True)
+    g, _ = synth.build(synth.GridSpec(nx, ny, 6, seed=7))
+    v = synth.make_velocity(g)
+    d = oracle.divergence(g, v)
+    dbs, *_ = oracle.bottom_rhs(g, v, d)
+    with NHSolver(g, options(precision=abi.FP64, bottom_solve=True)) as s:
+        s.attach_vgrid(v)
+        out = step(s, v)
+    c = oracle.op2d_coef(g)
+    b = dbs[1:-1, 1:-1].ravel(order="F")
+    x = out["pnh_bot"][1:-1, 1:-1].ravel(order="F")
+    xd = spl.spsolve(oracle.op2d_scipy(g, c).tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    assert out["rel_res"] <= 1e-10
+    _, it_cpu, _ = oracle.solve2d(g, c, b, rtol=1e-10)
+    assert abs(out["iterations"] - it_cpu) <= 8, (out["iterations"], it_cpu)
+
+
 def test_bottom_state_carries_over_and_warm_start():
     g, v1 = case("regional")
     v2 = copy.copy(v1)
