@@ -539,18 +539,34 @@ def bench_troneu(args):
 
 
 def mg_iteration_bytes(nx, ny):
-    """Bytes one iteration of the 2-D PCG streams. On the box: k2_dirmv reads z, d, the 5 coefficients
-    and S and writes d, q (10 words); k2_update reads x, d, r, q and writes x, r (6 words). Per cell of
-    every multigrid level: the first sweep reads r and K_cc and writes x (3 words), each of the other 3
-    sweeps reads r, x and K (5 planes) and writes y (8 words), the restriction reads r, x and K (7), the
-    prolongation reads and writes x and reads the coarse value (2): 36 words. Levels: the box's inner
-    rows, then 2 x 2 aggregates down to at most 64 unknowns."""
-    cells, a, b = nx * ny, nx - 2, ny - 2
+    """Bytes one iteration of the 2-D PCG streams (8-byte words). On the box: k2_dirmv reads z, d, the 5
+    coefficients and S and writes d, q (10 words); k2_update reads x, d, r, q and writes x, r (6 words).
+    Per cell of a multigrid level above the one-block tail: the way down (k_mg_down, one launch) reads r
+    and K (6 words) and writes the smoothed iterate and a quarter of a coarse right-hand side (1.25); the
+    way up reads the iterate, a quarter of a coarse value, r and K and writes the result (8.25, k_mg_up),
+    except on the box, whose way up is the prolongation (2.25) and two sweeps of 8 words. The tail (the
+    coarsest levels of at most 1024 cells each, at most 4 of them and 2048 cells together, never the box)
+    reads K (5 words per cell) and its first level's right-hand side, and writes that level's result.
+    Levels: the box, then 2 x 2 aggregates down to at most 64 unknowns."""
+    a, b = nx - 2, ny - 2
     levels = [nx * ny]
     while a * b > 64:
         a, b = (a + 1) // 2, (b + 1) // 2
         levels.append(a * b)
-    return 8 * (16 * cells + 36 * sum(levels[:-1]))
+    L = len(levels) - 1
+    T, tail = L + 1, 0
+    while T - 1 >= 1 and levels[T - 1] <= 1024 and L - T + 2 <= 4 and tail + levels[T - 1] <= 2048:
+        tail += levels[T - 1]
+        T -= 1
+    sep = min(T, L)  # levels with launches of their own
+    words = 16 * levels[0]
+    if sep >= 1:
+        words += 25.5 * levels[0] + sum(15.5 * levels[l] for l in range(1, sep))
+    if T <= L:
+        words += 5 * tail + 2 * levels[T]
+    else:
+        words += 2 * levels[L]  # the dense solve of the last level: its right-hand side in, its result out
+    return int(8 * words)
 
 
 def bench_bottom(args):
@@ -623,9 +639,10 @@ def bench_bottom(args):
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_iteration": mg_iteration_bytes(g.nx, g.ny),
-                     "note": "the PCG's 128 B per box cell plus 288 B per cell of every multigrid level; about "
-                             "36 launches per iteration on grids down to 64 x 48 cells (one block below that) make "
-                             "the solve launch-bound, "
+                     "note": "what the kernels stream: the PCG's 128 B per box cell, the fused level kernels' 124 B "
+                             "per level cell (204 on the box), the one-block tail's operator; about 17 launches per "
+                             "iteration (one per level and direction down to 64 x 48 cells, one block below that) "
+                             "make the solve launch-bound, "
                              "and its working set (about 130 MB) stays in the Infinity Cache"},
         "cpu_baseline": {"value": round((cpu_it / cpu_s) / per_solve, 4), "unit": "solves/s", "cores": 1,
                          "kind": "port",

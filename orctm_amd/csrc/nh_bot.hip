@@ -12,8 +12,8 @@
 //                   Galerkin operators, damped Jacobi smoothing, a dense solve on the last level),
 //                   warm-started from the previous solution as KSPSetInitialGuessNonzero does
 //                   (mo_petsc.f90:165), to ||b - A2 p|| <= rtol ||b|| checked every kBotChunk
-//                   iterations; block sums in fixed order and element-wise V-cycle kernels, so the
-//                   iterates are the same bits on every run
+//                   iterations; block sums in fixed order and V-cycle kernels whose cell values do
+//                   not depend on the blocking, so the iterates are the same bits on every run
 //   k2_scatter      PNH_bot on the interior, then bounds_exch(PNH_bot) (ocvtot_nh.f90:146)
 //   k_bot_correct   ComputeRHS's corrections of DIVG by PNH_bot at dry neighbours and the bottom face
 // FP64 throughout; the bottom values, DIVG_bot_surf and the corrections evaluate each expression left
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(BB) void k2_update(BDims g, double* __restrict__ x,
   }
 }
 
-// ---- the multigrid V-cycle (element-wise kernels; the host issues them level by level) ------------
+// ---- the multigrid V-cycle (the host issues the levels' kernels; the coarsest levels run in one block) --
 
 constexpr double kOmega = 2.0 / 3.0;  // damped Jacobi
 constexpr double kAlpha = 1.8;        // scale of every coarse-grid correction
@@ -288,14 +288,6 @@ struct MgDev {
   }
 };
 
-// x = omega r / K_cc on the unknowns, 0 elsewhere (the first sweep from x = 0)
-__global__ __launch_bounds__(BB) void k_mg_jac0(MgDev L, const double* __restrict__ r, double* __restrict__ x) {
-  const int c = blockIdx.x * BB + threadIdx.x;
-  if (c >= L.a * L.b) return;
-  const int i = c % L.a, j = c / L.a;
-  x[c] = L.active(i, j) ? kOmega * r[c] / L.k[c] : 0.0;
-}
-
 // y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere; with dot, the block sums of r y
 __global__ __launch_bounds__(BB) void k_mg_jac(MgDev L, const double* __restrict__ r, const double* __restrict__ x,
                                                double* __restrict__ y, double* __restrict__ part, int dot) {
@@ -311,24 +303,6 @@ __global__ __launch_bounds__(BB) void k_mg_jac(MgDev L, const double* __restrict
     y[c] = v;
   }
   if (dot) block_sums(ry, 0.0, 0.0, part);
-}
-
-// rc = the sum of r - K x over each aggregate (children in the order (0,0), (1,0), (0,1), (1,1))
-__global__ __launch_bounds__(BB) void k_mg_restrict(MgDev F, MgDev C, const double* __restrict__ r,
-                                                    const double* __restrict__ x, double* __restrict__ rc) {
-  const int c = blockIdx.x * BB + threadIdx.x;
-  if (c >= C.a * C.b) return;
-  const int I = c % C.a, J = c / C.a;
-  double s = 0.0;
-  for (int dj = 0; dj < 2; ++dj)
-    for (int di = 0; di < 2; ++di) {
-      const int i = F.o + 2 * I + di, j = F.o + 2 * J + dj;
-      if (F.active(i, j)) {
-        const int f = j * F.a + i;
-        s += r[f] - F.kx(x, i, j, f);
-      }
-    }
-  rc[c] = s;
 }
 
 // x += kAlpha xc of the cell's aggregate, on the unknowns
@@ -365,14 +339,147 @@ __global__ __launch_bounds__(BB) void k_mg_dot(int n, const double* __restrict__
   block_sums(c < n ? r[c] * z[c] : 0.0, 0.0, 0.0, part);
 }
 
+// ---- a level's way down in one launch -------------------------------------------------------------
+// With kNu = 2 the way down through a level is
+//   the first sweep from x = 0:  x = omega r / K_cc on the unknowns, 0 elsewhere;
+//   one more sweep (k_mg_jac):    y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere;
+//   the restriction:              rc = the sum of r - K y over each aggregate's unknowns, children in the
+//                                 order (0,0), (1,0), (0,1), (1,1).
+// k_mg_down does all three for a block of CTX x CTY aggregates: r and K of its cells and a ring of 2
+// go to LDS, the first sweep (pointwise) runs on all of them, the second on the cells and a ring of 1,
+// then the residuals of the cells and their aggregate sums follow. The first and last blocks of a row
+// or column of blocks also take the level's cells outside every aggregate. Only y goes to memory: x
+// is overwritten, unread, by the first sweep on the way up. Each value is computed by the same
+// operations in the same order whichever block computes it, so the results do not depend on the
+// blocking.
+constexpr int CTX = 16, CTY = 4;  // aggregates per block
+constexpr int EXW = 2 * CTX + 6, EXH = 2 * CTY + 6;  // LDS extents: the block's cells (at most 2 CT + 2) and the ring
+
+__device__ __forceinline__ int down_tiles_x(const MgDev& C) { return (C.a + CTX - 1) / CTX; }
+
+__global__ __launch_bounds__(BB) void k_mg_down(MgDev F, MgDev C, const double* __restrict__ r,
+                                                double* __restrict__ x1g, double* __restrict__ rc) {
+  static_assert(kNu == 2, "the fused way down runs exactly two sweeps");
+  __shared__ double sk[5][EXH][EXW], sr[EXH][EXW], s0[EXH][EXW], s1[EXH][EXW], se[EXH][EXW];
+  const int ntx = down_tiles_x(C), nty = (C.b + CTY - 1) / CTY;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int ilo = tx == 0 ? 0 : F.o + 2 * CTX * tx, ihi = tx == ntx - 1 ? F.a : F.o + 2 * CTX * (tx + 1);
+  const int jlo = ty == 0 ? 0 : F.o + 2 * CTY * ty, jhi = ty == nty - 1 ? F.b : F.o + 2 * CTY * (ty + 1);
+  const int ei = ilo - 2, ej = jlo - 2, ew = ihi - ilo + 4, eh = jhi - jlo + 4;
+  const size_t n = (size_t)F.a * F.b;
+  // K x at LDS cell (li, lj) = level cell (i, j), in MgDev::kx's order
+  auto kx = [&](double (*x)[EXW], int li, int lj, int i, int j) {
+    double s = sk[0][lj][li] * x[lj][li];
+    if (i > 0) s += sk[1][lj][li] * x[lj][li - 1];
+    if (i < F.a - 1) s += sk[2][lj][li] * x[lj][li + 1];
+    if (j > 0) s += sk[3][lj][li] * x[lj - 1][li];
+    if (j < F.b - 1) s += sk[4][lj][li] * x[lj + 1][li];
+    return s;
+  };
+  for (int e = threadIdx.x; e < ew * eh; e += BB) {
+    const int li = e % ew, lj = e / ew, i = ei + li, j = ej + lj;
+    if (i < 0 || j < 0 || i >= F.a || j >= F.b) continue;
+    const size_t c = (size_t)j * F.a + i;
+    const double rv = r[c];
+    sr[lj][li] = rv;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) sk[q][lj][li] = F.k[q * n + c];
+    s0[lj][li] = F.active(i, j) ? kOmega * rv / sk[0][lj][li] : 0.0;
+  }
+  __syncthreads();
+  const int w1 = ew - 2, h1 = eh - 2;  // the cells and a ring of 1
+  for (int e = threadIdx.x; e < w1 * h1; e += BB) {
+    const int li = 1 + e % w1, lj = 1 + e / w1, i = ei + li, j = ej + lj;
+    if (i < 0 || j < 0 || i >= F.a || j >= F.b) continue;
+    double v = 0.0;
+    if (F.active(i, j)) v = s0[lj][li] + kOmega * (sr[lj][li] - kx(s0, li, lj, i, j)) / sk[0][lj][li];
+    s1[lj][li] = v;
+  }
+  __syncthreads();
+  const int w0 = ew - 4, h0 = eh - 4;  // the block's own cells
+  for (int e = threadIdx.x; e < w0 * h0; e += BB) {
+    const int li = 2 + e % w0, lj = 2 + e / w0, i = ei + li, j = ej + lj;
+    x1g[(size_t)j * F.a + i] = s1[lj][li];
+    if (F.active(i, j)) se[lj][li] = sr[lj][li] - kx(s1, li, lj, i, j);
+  }
+  __syncthreads();
+  const int I1 = min(C.a, CTX * (tx + 1)), J1 = min(C.b, CTY * (ty + 1));
+  for (int e = threadIdx.x; e < CTX * CTY; e += BB) {
+    const int I = CTX * tx + e % CTX, J = CTY * ty + e / CTX;
+    if (I >= I1 || J >= J1) continue;
+    double s = 0.0;
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        const int i = F.o + 2 * I + di, j = F.o + 2 * J + dj;
+        if (F.active(i, j)) s += se[j - ej][i - ei];
+      }
+    rc[(size_t)J * C.a + I] = s;
+  }
+}
+
+// A level's way up in one launch (levels below the box; the box's last sweep also forms the r.z block
+// sums and keeps its own kernels): the coarse correction (k_mg_prolong) and kNu = 2 sweeps (k_mg_jac)
+// for a block of 2 CTX x 2 CTY cells, with r, K and the corrected iterate of its cells and a ring of 2
+// in LDS, the first sweep on the cells and a ring of 1, the second on the cells, written to `out`
+// (not to the iterate read, which other blocks still read). Each cell's arithmetic is that of
+// k_mg_prolong and k_mg_jac, so the results are the same bits as with those kernels launched.
+__global__ __launch_bounds__(BB) void k_mg_up(MgDev F, MgDev C, const double* __restrict__ r,
+                                              const double* __restrict__ x, const double* __restrict__ xc,
+                                              double* __restrict__ out) {
+  static_assert(kNu == 2, "the fused way up runs exactly two sweeps");
+  constexpr int TW = 2 * CTX, TH = 2 * CTY;
+  __shared__ double sk[5][EXH][EXW], sr[EXH][EXW], sa[EXH][EXW], sb[EXH][EXW];
+  const int ntx = (F.a + TW - 1) / TW;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int ilo = TW * tx, ihi = min(F.a, ilo + TW), jlo = TH * ty, jhi = min(F.b, jlo + TH);
+  const int ei = ilo - 2, ej = jlo - 2, ew = ihi - ilo + 4, eh = jhi - jlo + 4;
+  const size_t n = (size_t)F.a * F.b;
+  auto kx = [&](double (*v)[EXW], int li, int lj, int i, int j) {
+    double s = sk[0][lj][li] * v[lj][li];
+    if (i > 0) s += sk[1][lj][li] * v[lj][li - 1];
+    if (i < F.a - 1) s += sk[2][lj][li] * v[lj][li + 1];
+    if (j > 0) s += sk[3][lj][li] * v[lj - 1][li];
+    if (j < F.b - 1) s += sk[4][lj][li] * v[lj + 1][li];
+    return s;
+  };
+  for (int e = threadIdx.x; e < ew * eh; e += BB) {
+    const int li = e % ew, lj = e / ew, i = ei + li, j = ej + lj;
+    if (i < 0 || j < 0 || i >= F.a || j >= F.b) continue;
+    const size_t c = (size_t)j * F.a + i;
+    sr[lj][li] = r[c];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) sk[q][lj][li] = F.k[q * n + c];
+    double v = x[c];
+    if (F.active(i, j)) v += kAlpha * xc[((j - F.o) / 2) * C.a + (i - F.o) / 2];
+    sa[lj][li] = v;
+  }
+  __syncthreads();
+  const int w1 = ew - 2, h1 = eh - 2;
+  for (int e = threadIdx.x; e < w1 * h1; e += BB) {
+    const int li = 1 + e % w1, lj = 1 + e / w1, i = ei + li, j = ej + lj;
+    if (i < 0 || j < 0 || i >= F.a || j >= F.b) continue;
+    double v = 0.0;
+    if (F.active(i, j)) v = sa[lj][li] + kOmega * (sr[lj][li] - kx(sa, li, lj, i, j)) / sk[0][lj][li];
+    sb[lj][li] = v;
+  }
+  __syncthreads();
+  const int w0 = ew - 4, h0 = eh - 4;
+  for (int e = threadIdx.x; e < w0 * h0; e += BB) {
+    const int li = 2 + e % w0, lj = 2 + e / w0, i = ei + li, j = ej + lj;
+    double v = 0.0;
+    if (F.active(i, j)) v = sb[lj][li] + kOmega * (sr[lj][li] - kx(sb, li, lj, i, j)) / sk[0][lj][li];
+    out[(size_t)j * F.a + i] = v;
+  }
+}
+
 // ---- the coarse tail of the V-cycle in one block ---------------------------------------------------
 // Below a few hundred cells a level's launches cost far more than its work. The last levels of the
 // hierarchy (each at most kTailCells cells, halo included: one cell per thread) therefore run in one
 // block: each thread keeps its cell's 5 operator entries in registers, the iterates, right-hand sides
 // and residuals of every tail level sit in LDS, and the phases the separate kernels would run are
-// separated by block barriers. Every cell's arithmetic is the separate kernels' (k_mg_jac0, k_mg_jac,
-// k_mg_restrict split into the cell residuals and the same 4-child sums in the same order,
-// k_mg_prolong, k_mg_dense), so the V-cycle gives the same bits.
+// separated by block barriers. Every cell's arithmetic is that of the level kernels (the sweeps and
+// the restriction of k_mg_down, k_mg_prolong, k_mg_dense), so the V-cycle gives the same bits as with
+// every level launched on its own.
 constexpr int kTailCells = 1024;
 constexpr int kTailLevels = 4;
 constexpr size_t kTailLds = 64 * 1024;  // LDS bytes the tail may use (4 doubles per cell)
@@ -662,18 +769,11 @@ hipError_t launch_vcycle(hipStream_t st, BotState& b) {
     const BotState::MgLevel& lv = b.mg[l];
     const MgDev D = mgdev(lv);
     const double* r = l == 0 ? b.r : lv.r;
-    double *xa = lv.x, *xb = lv.y;
-    const unsigned nb = nblocks((long long)lv.a * lv.b);
-    hipLaunchKernelGGL(k_mg_jac0, dim3(nb), dim3(BB), 0, st, D, r, xa);
-    BCHK(hipGetLastError());
-    for (int s = 1; s < kNu; ++s) {
-      hipLaunchKernelGGL(k_mg_jac, dim3(nb), dim3(BB), 0, st, D, r, (const double*)xa, xb, b.part, 0);
-      BCHK(hipGetLastError());
-      std::swap(xa, xb);
-    }
+    // the sweeps' ping-pong between the level's two buffers: the way down leaves its iterate in y
+    double *xa = lv.y, *xb = lv.x;
     const BotState::MgLevel& cv = b.mg[l + 1];
-    hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((long long)cv.a * cv.b)), dim3(BB), 0, st, D, mgdev(cv), r,
-                       (const double*)xa, cv.r);
+    const unsigned nbk = (unsigned)(((cv.a + CTX - 1) / CTX) * ((cv.b + CTY - 1) / CTY));
+    hipLaunchKernelGGL(k_mg_down, dim3(nbk), dim3(BB), 0, st, D, mgdev(cv), r, xa, cv.r);
     BCHK(hipGetLastError());
     xf[l] = xa;
     xo[l] = xb;
@@ -727,6 +827,14 @@ hipError_t launch_vcycle(hipStream_t st, BotState& b) {
     const BotState::MgLevel& lv = b.mg[l];
     const MgDev D = mgdev(lv);
     const double* r = l == 0 ? b.r : lv.r;
+    if (l >= 1) {
+      const unsigned nbk = (unsigned)(((lv.a + 2 * CTX - 1) / (2 * CTX)) * ((lv.b + 2 * CTY - 1) / (2 * CTY)));
+      hipLaunchKernelGGL(k_mg_up, dim3(nbk), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), r, (const double*)xf[l],
+                         (const double*)xf[l + 1], xo[l]);
+      BCHK(hipGetLastError());
+      xf[l] = xo[l];
+      continue;
+    }
     const unsigned nb = nblocks((long long)lv.a * lv.b);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), (const double*)xf[l + 1],
                        xf[l]);
