@@ -543,8 +543,8 @@ def mg_iteration_bytes(nx, ny):
     coefficients and S and writes d, q (10 words); k2_update reads x, d, r, q and writes x, r (6 words).
     Per cell of a multigrid level above the one-block tail: the way down (k_mg_down, one launch) reads r
     and K (6 words) and writes the smoothed iterate and a quarter of a coarse right-hand side (1.25); the
-    way up reads the iterate, a quarter of a coarse value, r and K and writes the result (8.25, k_mg_up),
-    except on the box, whose way up is the prolongation (2.25) and two sweeps of 8 words. The tail (the
+    way up reads the iterate, a quarter of a coarse value, r and K and writes the result (8.25, k_mg_up);
+    on the box k_mg_dotm then reads r and z for the r.z block sums (2). The tail (the
     coarsest levels of at most 1024 cells each, at most 4 of them and 2048 cells together, never the box)
     reads K (5 words per cell) and its first level's right-hand side, and writes that level's result.
     Levels: the box, then 2 x 2 aggregates down to at most 64 unknowns."""
@@ -561,7 +561,7 @@ def mg_iteration_bytes(nx, ny):
     sep = min(T, L)  # levels with launches of their own
     words = 16 * levels[0]
     if sep >= 1:
-        words += 25.5 * levels[0] + sum(15.5 * levels[l] for l in range(1, sep))
+        words += 17.5 * levels[0] + sum(15.5 * levels[l] for l in range(1, sep))
     if T <= L:
         words += 5 * tail + 2 * levels[T]
     else:
@@ -640,7 +640,7 @@ def bench_bottom(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_iteration": mg_iteration_bytes(g.nx, g.ny),
                      "note": "what the kernels stream: the PCG's 128 B per box cell, the fused level kernels' 124 B "
-                             "per level cell (204 on the box), the one-block tail's operator; about 17 launches per "
+                             "per level cell (140 on the box), the one-block tail's operator; about 16 launches per "
                              "iteration (one per level and direction down to 64 x 48 cells, one block below that) "
                              "make the solve launch-bound, "
                              "and its working set (about 130 MB) stays in the Infinity Cache"},

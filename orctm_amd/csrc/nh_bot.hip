@@ -268,6 +268,15 @@ __global__ __launch_bounds__(BB) void k2_update(BDims g, double* __restrict__ x,
 }
 
 // ---- the multigrid V-cycle (the host issues the levels' kernels; the coarsest levels run in one block) --
+// The operations on a level's cells (K the level's operator, unknowns the cells with MgDev::active):
+//   first sweep from x = 0:  x = omega r / K_cc on the unknowns, 0 elsewhere
+//   sweep:                   y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere
+//   restriction:             rc = the sum of r - K x over an aggregate's unknowns, children in the order
+//                            (0,0), (1,0), (0,1), (1,1)
+//   coarse correction:       x += kAlpha xc of the cell's aggregate, on the unknowns
+//   dense solve:             x = inv r over the last level's unknowns (numbered row by row), 0 elsewhere
+// K x is MgDev::kx (centre, west, east, north, south). Whichever kernel computes a value computes it by
+// these operations in this order, so the V-cycle's results do not depend on how it is launched.
 
 constexpr double kOmega = 2.0 / 3.0;  // damped Jacobi
 constexpr double kAlpha = 1.8;        // scale of every coarse-grid correction
@@ -287,32 +296,6 @@ struct MgDev {
     return s;
   }
 };
-
-// y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere; with dot, the block sums of r y
-__global__ __launch_bounds__(BB) void k_mg_jac(MgDev L, const double* __restrict__ r, const double* __restrict__ x,
-                                               double* __restrict__ y, double* __restrict__ part, int dot) {
-  const int c = blockIdx.x * BB + threadIdx.x;
-  double ry = 0.0;
-  if (c < L.a * L.b) {
-    const int i = c % L.a, j = c / L.a;
-    double v = 0.0;
-    if (L.active(i, j)) {
-      v = x[c] + kOmega * (r[c] - L.kx(x, i, j, c)) / L.k[c];
-      ry = r[c] * v;
-    }
-    y[c] = v;
-  }
-  if (dot) block_sums(ry, 0.0, 0.0, part);
-}
-
-// x += kAlpha xc of the cell's aggregate, on the unknowns
-__global__ __launch_bounds__(BB) void k_mg_prolong(MgDev F, MgDev C, const double* __restrict__ xc,
-                                                   double* __restrict__ x) {
-  const int c = blockIdx.x * BB + threadIdx.x;
-  if (c >= F.a * F.b) return;
-  const int i = c % F.a, j = c / F.a;
-  if (F.active(i, j)) x[c] += kAlpha * xc[((j - F.o) / 2) * C.a + (i - F.o) / 2];
-}
 
 // x = inv r over the level's unknowns (numbered row by row), 0 elsewhere; one block
 __global__ __launch_bounds__(BB) void k_mg_dense(MgDev L, int n, const double* __restrict__ inv,
@@ -340,12 +323,8 @@ __global__ __launch_bounds__(BB) void k_mg_dot(int n, const double* __restrict__
 }
 
 // ---- a level's way down in one launch -------------------------------------------------------------
-// With kNu = 2 the way down through a level is
-//   the first sweep from x = 0:  x = omega r / K_cc on the unknowns, 0 elsewhere;
-//   one more sweep (k_mg_jac):    y = x + omega (r - K x) / K_cc on the unknowns, 0 elsewhere;
-//   the restriction:              rc = the sum of r - K y over each aggregate's unknowns, children in the
-//                                 order (0,0), (1,0), (0,1), (1,1).
-// k_mg_down does all three for a block of CTX x CTY aggregates: r and K of its cells and a ring of 2
+// With kNu = 2 the way down through a level is the first sweep from x = 0, one more sweep (y) and the
+// restriction of y. k_mg_down does all three for a block of CTX x CTY aggregates: r and K of its cells and a ring of 2
 // go to LDS, the first sweep (pointwise) runs on all of them, the second on the cells and a ring of 1,
 // then the residuals of the cells and their aggregate sums follow. The first and last blocks of a row
 // or column of blocks also take the level's cells outside every aggregate. Only y goes to memory: x
@@ -417,12 +396,10 @@ __global__ __launch_bounds__(BB) void k_mg_down(MgDev F, MgDev C, const double* 
   }
 }
 
-// A level's way up in one launch (levels below the box; the box's last sweep also forms the r.z block
-// sums and keeps its own kernels): the coarse correction (k_mg_prolong) and kNu = 2 sweeps (k_mg_jac)
-// for a block of 2 CTX x 2 CTY cells, with r, K and the corrected iterate of its cells and a ring of 2
-// in LDS, the first sweep on the cells and a ring of 1, the second on the cells, written to `out`
-// (not to the iterate read, which other blocks still read). Each cell's arithmetic is that of
-// k_mg_prolong and k_mg_jac, so the results are the same bits as with those kernels launched.
+// A level's way up in one launch: the coarse correction and kNu = 2 sweeps for a block of 2 CTX x 2 CTY
+// cells, with r, K and the corrected iterate of its cells and a ring of 2 in LDS, the first sweep on
+// the cells and a ring of 1, the second on the cells, written to `out` (not to the iterate read, which
+// other blocks still read). On the box, `out` is z and k_mg_dotm forms the r.z block sums after it.
 __global__ __launch_bounds__(BB) void k_mg_up(MgDev F, MgDev C, const double* __restrict__ r,
                                               const double* __restrict__ x, const double* __restrict__ xc,
                                               double* __restrict__ out) {
@@ -472,14 +449,24 @@ __global__ __launch_bounds__(BB) void k_mg_up(MgDev F, MgDev C, const double* __
   }
 }
 
+// the block sums of r z over the box's unknowns (0 elsewhere), in blocks of BB consecutive cells, for
+// the k2_reduce that follows (the same block layout as k2_dirmv's sums)
+__global__ __launch_bounds__(BB) void k_mg_dotm(MgDev L, const double* __restrict__ r, const double* __restrict__ z,
+                                                double* __restrict__ part) {
+  const int c = blockIdx.x * BB + threadIdx.x;
+  double ry = 0.0;
+  if (c < L.a * L.b && L.active(c % L.a, c / L.a)) ry = r[c] * z[c];
+  block_sums(ry, 0.0, 0.0, part);
+}
+
 // ---- the coarse tail of the V-cycle in one block ---------------------------------------------------
 // Below a few hundred cells a level's launches cost far more than its work. The last levels of the
 // hierarchy (each at most kTailCells cells, halo included: one cell per thread) therefore run in one
 // block: each thread keeps its cell's 5 operator entries in registers, the iterates, right-hand sides
 // and residuals of every tail level sit in LDS, and the phases the separate kernels would run are
-// separated by block barriers. Every cell's arithmetic is that of the level kernels (the sweeps and
-// the restriction of k_mg_down, k_mg_prolong, k_mg_dense), so the V-cycle gives the same bits as with
-// every level launched on its own.
+// separated by block barriers. Every value is computed by the V-cycle's operations in their order (the
+// restriction as the cell residuals, then the same 4-child sums), so the results are those of the
+// level kernels.
 constexpr int kTailCells = 1024;
 constexpr int kTailLevels = 4;
 constexpr size_t kTailLds = 64 * 1024;  // LDS bytes the tail may use (4 doubles per cell)
@@ -827,28 +814,17 @@ hipError_t launch_vcycle(hipStream_t st, BotState& b) {
     const BotState::MgLevel& lv = b.mg[l];
     const MgDev D = mgdev(lv);
     const double* r = l == 0 ? b.r : lv.r;
-    if (l >= 1) {
-      const unsigned nbk = (unsigned)(((lv.a + 2 * CTX - 1) / (2 * CTX)) * ((lv.b + 2 * CTY - 1) / (2 * CTY)));
-      hipLaunchKernelGGL(k_mg_up, dim3(nbk), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), r, (const double*)xf[l],
-                         (const double*)xf[l + 1], xo[l]);
-      BCHK(hipGetLastError());
-      xf[l] = xo[l];
-      continue;
-    }
-    const unsigned nb = nblocks((long long)lv.a * lv.b);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), (const double*)xf[l + 1],
-                       xf[l]);
+    double* out = l == 0 ? b.z : xo[l];
+    const unsigned nbk = (unsigned)(((lv.a + 2 * CTX - 1) / (2 * CTX)) * ((lv.b + 2 * CTY - 1) / (2 * CTY)));
+    hipLaunchKernelGGL(k_mg_up, dim3(nbk), dim3(BB), 0, st, D, mgdev(b.mg[l + 1]), r, (const double*)xf[l],
+                       (const double*)xf[l + 1], out);
     BCHK(hipGetLastError());
-    double *xa = xf[l], *xb = xo[l];
-    for (int s = 1; s <= kNu; ++s) {
-      const bool last = l == 0 && s == kNu;
-      double* out = last ? b.z : xb;
-      hipLaunchKernelGGL(k_mg_jac, dim3(nb), dim3(BB), 0, st, D, r, (const double*)xa, out, b.part, last ? 1 : 0);
+    xf[l] = out;
+    if (l == 0) {
+      hipLaunchKernelGGL(k_mg_dotm, dim3(nblocks((long long)lv.a * lv.b)), dim3(BB), 0, st, D, (const double*)b.r,
+                         (const double*)b.z, b.part);
       BCHK(hipGetLastError());
-      xb = xa;
-      xa = out;
     }
-    xf[l] = xa;
   }
   return hipSuccess;
 }
