@@ -331,7 +331,7 @@ __global__ __launch_bounds__(BB) void k_mg_dot(int n, const double* __restrict__
 // is overwritten, unread, by the first sweep on the way up. Each value is computed by the same
 // operations in the same order whichever block computes it, so the results do not depend on the
 // blocking.
-constexpr int CTX = 16, CTY = 4;  // aggregates per block
+constexpr int CTX = 8, CTY = 4;  // aggregates per block (the fastest of 4-16 x 2-8 on a 1024 x 768 box)
 constexpr int EXW = 2 * CTX + 6, EXH = 2 * CTY + 6;  // LDS extents: the block's cells (at most 2 CT + 2) and the ring
 
 __device__ __forceinline__ int down_tiles_x(const MgDev& C) { return (C.a + CTX - 1) / CTX; }
