@@ -10,6 +10,10 @@
 ! the divergence, with CG2D4NONHY the 2-D bottom-pressure solve and the corrections of DIVG, the solve
 ! and the projection then all run on the device; and
 ! troneu_device replaces CALL TRONEU (orctm.f90:1414): the barotropic SOR solve on the device.
+! aufw_nh / aufr_nh save and restore the solver's resident pressure, from which the next solve
+! warm-starts, with the restart files: call aufw_nh after AUFW writes its records and aufr_nh after
+! AUFR and petsc_start (the reference's restart holds no NH state, aufw_fragment.f90:40-47, and
+! beleg_zero.f90:440-506 zeroes it, so its first solve after a restart starts from zero).
 ! The module reads the same module globals compute_matrix reads (MO_PARAM1 sizes, MO_COMMO1 metrics
 ! and masks, MO_OBCS flags) and hands the caller's arrays, halo included, to the C ABI.
 ! One GPU holds the whole domain (nprocx = nprocy = 1). Build (with the model's -r8 or its equivalent):
@@ -18,10 +22,12 @@ MODULE orctm_nh_c
   USE, INTRINSIC :: ISO_C_BINDING
   IMPLICIT NONE
   PRIVATE
-  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end, ocvtot_nh_device, troneu_device
+  PUBLIC :: petsc_start, petsc_solve_3D, petsc_end, ocvtot_nh_device, troneu_device, aufw_nh, aufr_nh
   PUBLIC :: orctm_nh_grid_t, orctm_nh_options_t, orctm_nh_stats_t, orctm_nh_vgrid_t
 
   INTEGER(C_INT), PARAMETER, PUBLIC :: ORCTM_NH_OK = 0, ORCTM_NH_FP64 = 0, ORCTM_NH_MIXED = 1
+  ! code of the resident-pressure record in a restart file (EXTRA-style header: IDATE, ICODE, KCODE, IEDIM)
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: NH_RESTART_CODE = 594
 
   TYPE, BIND(C) :: orctm_nh_grid_t
     INTEGER(C_INT32_T) :: ie, je, ke, i_start, j_start
@@ -117,6 +123,16 @@ MODULE orctm_nh_c
       REAL(C_DOUBLE), INTENT(IN) :: divg(*)
       REAL(C_DOUBLE), INTENT(INOUT) :: pnh(*)
       TYPE(orctm_nh_stats_t), INTENT(OUT) :: stats
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_get_solution(handle, pnh) BIND(C, name="orctm_nh_get_solution")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(INOUT) :: pnh(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION orctm_nh_set_guess(handle, pnh) BIND(C, name="orctm_nh_set_guess")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(IN) :: pnh(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION orctm_nh_destroy(handle) BIND(C, name="orctm_nh_destroy")
       IMPORT :: C_INT, C_PTR
@@ -239,6 +255,45 @@ CONTAINS
     ierr = orctm_nh_troneu(sor_handle, U1O, UZO, V1E, VZE, B1O, Z1O, C_NULL_PTR)
     IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_troneu status ', ierr
   END SUBROUTINE troneu_device
+
+  ! The resident pressure (the interior of PNH from the last solve, from which the next solve warm-starts)
+  ! as two records on the restart unit IUNIT, sequential and unformatted like the restart files
+  ! (aufw_fragment.f90:38-47): an EXTRA-style header IDATE, ICODE = NH_RESTART_CODE, KCODE = KE,
+  ! IEDIM = IE*JE, then the (IE,JE,KE) field with a zero halo.
+  SUBROUTINE aufw_nh(iunit, idate)
+    USE MO_PARAM1, ONLY: IE, JE, KE
+    INTEGER, INTENT(IN) :: iunit
+    INTEGER(C_INT32_T), INTENT(IN) :: idate
+    REAL(C_DOUBLE), ALLOCATABLE :: p(:,:,:)
+    INTEGER(C_INT) :: ierr
+    ALLOCATE(p(IE, JE, KE))
+    p(:,:,:) = 0.0_C_DOUBLE
+    ierr = orctm_nh_get_solution(nh_handle, p)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_get_solution status ', ierr
+    WRITE(iunit) idate, NH_RESTART_CODE, INT(KE, C_INT32_T), INT(IE * JE, C_INT32_T)
+    WRITE(iunit) p
+    DEALLOCATE(p)
+  END SUBROUTINE aufw_nh
+
+  ! Reads the two records aufw_nh wrote (after petsc_start) and makes their field the initial guess of
+  ! the next solve, which then warm-starts exactly as it would have without the restart (mo_petsc.f90:232).
+  SUBROUTINE aufr_nh(iunit)
+    USE MO_PARAM1, ONLY: IE, JE, KE
+    INTEGER, INTENT(IN) :: iunit
+    INTEGER(C_INT32_T) :: hdr(4)
+    REAL(C_DOUBLE), ALLOCATABLE :: p(:,:,:)
+    INTEGER(C_INT) :: ierr
+    READ(iunit) hdr
+    IF (hdr(2) /= NH_RESTART_CODE .OR. hdr(3) /= KE .OR. hdr(4) /= IE * JE) THEN
+      PRINT *, 'aufr_nh: not a resident-pressure record of this grid: ', hdr
+      RETURN
+    ENDIF
+    ALLOCATE(p(IE, JE, KE))
+    READ(iunit) p
+    ierr = orctm_nh_set_guess(nh_handle, p)
+    IF (ierr /= ORCTM_NH_OK) PRINT *, 'orctm_nh_set_guess status ', ierr
+    DEALLOCATE(p)
+  END SUBROUTINE aufr_nh
 
   ! mo_petsc.f90:131-143
   SUBROUTINE petsc_end

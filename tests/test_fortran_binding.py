@@ -33,7 +33,7 @@ def test_binding_compiles_and_links(tmp_path):
     if not os.path.exists(abi.LIB_PATH):
         pytest.skip("liborctm_nh.so is not built")
     built = fb.build(out=str(tmp_path))
-    assert set(built) == {"plain", "cg2d"}
+    assert set(built) == {"plain", "cg2d", "restart"}
     for exe in built.values():
         assert os.access(exe, os.X_OK)
 

@@ -71,3 +71,40 @@ def test_fortran_host_matches_the_c_abi(variant, tmp_path):
     p1, p2 = got[0], got[1]
     assert np.abs(p1).max() > 0
     assert np.linalg.norm(p2 - p1) <= 1e-8 * np.linalg.norm(p1)
+
+
+def test_restart_restores_the_warm_start(tmp_path):
+    """aufw_nh saves the resident pressure with the restart files and aufr_nh gives it back to a new run:
+    the restarted run's next solve is bit-identical to the uninterrupted run's (which warm-starts from
+    that pressure, mo_petsc.f90:232), where a restart without it starts from zero and takes a different
+    path to the same tolerance. The record is a Fortran sequential header (IDATE, code 594, KE, IE*JE)
+    and the (IE,JE,KE) field: the first solve's interior, zero halo."""
+    exe = os.path.join(FBUILD, "restart", "orctm_nh_frestart")
+    if not os.path.exists(exe):
+        pytest.skip("the Fortran restart host is not built (no amdflang when the tree was built)")
+    print("[fortran] restart", flush=True)
+    g, divg, v, b, b1o = inputs()
+    fin, frst, fout = str(tmp_path / "in.bin"), str(tmp_path / "restart.bin"), str(tmp_path / "out.bin")
+    write_input(fin, g, divg, v, b, b1o)
+    r = subprocess.run([exe, fin, frst, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "done" in r.stdout and "status" not in r.stdout and "aufr_nh" not in r.stdout, r.stdout
+    shape = (g.IE, g.JE, g.KE)
+    uninterrupted, restarted, cold = read_output(fout, [shape] * 3)
+    assert bits_equal(restarted, uninterrupted)
+    assert not bits_equal(cold, uninterrupted)
+    assert np.linalg.norm(cold - uninterrupted) <= 1e-7 * np.linalg.norm(uninterrupted)
+    raw = open(frst, "rb").read()
+    n = g.IE * g.JE * g.KE
+    m1 = np.frombuffer(raw[0:4], np.int32)[0]
+    hdr = np.frombuffer(raw[4:20], np.int32)
+    m2 = np.frombuffer(raw[20:24], np.int32)[0]
+    m3 = np.frombuffer(raw[24:28], np.int32)[0]
+    field = np.frombuffer(raw[28:28 + 8 * n], np.float64).reshape(shape, order="F")
+    m4 = np.frombuffer(raw[28 + 8 * n:32 + 8 * n], np.int32)[0]
+    assert (m1, m2, m3, m4) == (16, 16, 8 * n, 8 * n) and len(raw) == 32 + 8 * n
+    assert list(hdr) == [20261203, 594, g.KE, g.IE * g.JE]
+    with NHSolver(g, options()) as s:
+        pnh = np.zeros(shape, order="F")
+        assert s.solve(divg, pnh)["rc"] == abi.OK
+    assert bits_equal(field, pnh)
