@@ -106,17 +106,52 @@ def time_solves(solver, steps, warmup, profiled=True):
     return wall, lat, stats, (solver.profile(reset=True) if profiled else None)
 
 
-def cpu_baseline(g, divg, seconds_hint=30.0):
+def host_cpu():
+    """The host's CPU model (first "model name" of /proc/cpuinfo) and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return model, n
+
+
+def cpu_baseline(g, divg, reps=5):
+    """The oracle's FP64 PCG + line preconditioner (C, OpenMP) on the host cores: one untimed solve, then
+    `reps` solves from a zero guess, each timed over its iterations only (the set-up of the inner operator,
+    the line factors and the work arrays excluded); value = 1 / median. The reference's own method,
+    FGMRES(30) with the line preconditioner standing in for PCMG(1 level), is timed once beside it.
+    Returns the baseline object and the oracle's FP64 solution."""
     import oracle
 
     A, S = oracle.assemble(g)
     b = oracle.rhs(g, divg)
-    t = time.perf_counter()
-    p, st = oracle.solve(g, A, S, b, precision=0)
-    dt = time.perf_counter() - t
-    return {"value": round(1.0 / dt, 6), "unit": "solves/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"one FP64 PCG + line-preconditioner solve of the full {g.nx}x{g.ny}x{g.nz} tile by the "
-                      f"C oracle (OpenMP), rel.res {st.rel_res:.1e} in {st.inner_iters} iterations, {dt:.1f} s"}
+    p, st = oracle.solve(g, A, S, b, precision=0)  # warm-up: first touch of every work array
+    times = []
+    for _ in range(reps):
+        p, st = oracle.solve(g, A, S, b, precision=0)
+        times.append(st.solve_ms)
+    xf, sf = oracle.fgmres(g, A, S, b, restart=30, rtol=1e-10)
+    med = statistics.median(times)
+    model, ncpu = host_cpu()
+    base = {"value": round(1e3 / med, 6), "unit": "solves/s", "cores": oracle.num_threads(), "kind": "port",
+            "host_cpu": model, "host_cpus_available": ncpu,
+            "sample": f"median of {reps} warm FP64 PCG + line-preconditioner solves of the full {g.nx}x{g.ny}x{g.nz} "
+                      f"tile by the C oracle (OpenMP), set-up excluded: {', '.join(f'{t / 1e3:.2f}' for t in times)} s, "
+                      f"{st.inner_iters} iterations to rel.res {st.rel_res:.1e}",
+            "fgmres30": {"value": round(1e3 / sf.solve_ms, 6), "unit": "solves/s", "iterations": sf.inner_iters,
+                         "rel_res": sf.rel_res, "seconds": round(sf.solve_ms / 1e3, 3),
+                         "note": "the reference's KSP type, FGMRES(30), with the line preconditioner in place of "
+                                 "PCMG(1 level); one warm solve, set-up excluded"}}
+    return base, p, A, b
 
 
 def main():
@@ -126,10 +161,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="regional", choices=["regional", "batch", "transect", "serve", "troneu",
                                                                      "bottom"])
-    ap.add_argument("--requests", type=int, default=24, help="serve workload: number of arrivals served")
-    ap.add_argument("--pool", type=int, default=6, help="serve workload: grid sizes pre-assembled")
+    ap.add_argument("--requests", type=int, default=48, help="serve workload: arrivals of the burst phase")
+    ap.add_argument("--requests2", type=int, default=40, help="serve workload: arrivals of the sustained phase")
+    ap.add_argument("--pool", type=int, default=7, help="serve workload: grid sizes pre-assembled")
     ap.add_argument("--rate", type=float, default=200.0, help="serve workload: Poisson arrival rate (1/s)")
+    ap.add_argument("--lanes", type=int, default=4, help="serve workload: server lanes")
     ap.add_argument("--nsys", type=int, default=64, help="batch workload: number of systems")
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
+                    help="batch workload: the precisions run (both by default; one for a short profile)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink nx, ny (debug only)")
     ap.add_argument("--tile-rows", type=int, default=0, help="inner-kernel tile height: 0 = tuned, or 4, 8, 12")
@@ -212,7 +251,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-inner/fp64-refinement",
+        "dtype": "fp32-inner (fp16 line factors)/fp64-refinement",
         "data": "synthetic (seeded generator, regional-tile recipe)",
         "config": {"workload": f"regional tile {spec.nx}x{spec.ny}x{spec.nz}, 25% land, partial cells, "
                                "mixed FP32 PCG + line preconditioner in FP64 refinement to rel.res 1e-10",
@@ -254,8 +293,20 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle FP64 solve on the host cores) ...")
-        line["cpu_baseline"] = cpu_baseline(g, divg)
+        log("[bench] CPU baseline (oracle FP64 solves on the host cores) ...")
+        import oracle
+
+        base, p_oracle, A, b = cpu_baseline(g, divg)
+        line["cpu_baseline"] = base
+        # the device solutions against the oracle: its FP64 solution and its FP64 operator's residual
+        nref = float(np.linalg.norm(p_oracle))
+        line["vs_oracle"] = {
+            "rel_err_mixed": float(np.linalg.norm(sols["mixed"] - p_oracle) / nref),
+            "rel_err_fp64": float(np.linalg.norm(sols["fp64"] - p_oracle) / nref),
+            "oracle_rel_res_mixed": oracle.rel_res(g, A, sols["mixed"], b),
+            "oracle_rel_res_fp64": oracle.rel_res(g, A, sols["fp64"], b),
+            "note": "||p - p_oracle|| / ||p_oracle|| with p_oracle the oracle's FP64 PCG solution to rel.res 1e-10; "
+                    "rel.res of the device solutions by the oracle's FP64 operator"}
     print(json.dumps(line), flush=True)
 
 
@@ -277,7 +328,10 @@ def bench_batch(args):
     cells = sum(g.cells for g, _ in systems)
     log(f"[bench] batch: {len(systems)} systems, {cells} cells, generated in {time.perf_counter() - t:.1f} s")
     out = {}
-    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+    precs = (("mixed", abi.MIXED), ("fp64", abi.FP64))
+    if args.precision != "both":
+        precs = tuple(x for x in precs if x[0] == args.precision)
+    for name, prec in precs:
         solvers = []
         for g, divg in systems:
             s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table))
@@ -330,7 +384,9 @@ def bench_batch(args):
         batch.close()
         for s in solvers:
             s.close()
-    mx, f64 = out["mixed"], out["fp64"]
+    mx, f64 = out.get("mixed"), out.get("fp64")
+    if mx is None:  # FP64 alone: report it in the mixed fields' place
+        mx, f64 = f64, None
     n = len(systems)
     prof = mx["prof"]
     # bytes moved by the matvec launches of one profiled batch solve / their summed device time
@@ -338,16 +394,18 @@ def bench_batch(args):
     line = {
         "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-inner (fp16 line factors)/fp64-refinement",
         "data": "synthetic (seeded generator, regional-tile recipe per system)",
         "config": {"workload": f"batch of {n} systems, nx,ny~U{{96..512}}, nz~U{{30..60}}, mixed, rel.res 1e-10",
                    "cells": cells},
-        "fp64_solves_per_s": round(n * args.steps / f64["wall"], 3),
-        "mixed_over_fp64": round(f64["wall"] / mx["wall"], 3),
+        "precisions_run": [name for name, _ in precs],
+        "fp64_solves_per_s": round(n * args.steps / f64["wall"], 3) if f64 else None,
+        "mixed_over_fp64": round(f64["wall"] / mx["wall"], 3) if f64 else None,
         "one_by_one_solves_per_s_mixed": round(n / mx["seq"], 3),
         "cell_iters_per_s": round(mx["cell_iters"] * args.steps / mx["wall"], 1),
-        "max_rel_res_mixed": mx["rel"], "max_rel_res_fp64": f64["rel"],
-        "tile_rows_mixed": mx["heights"], "tile_rows_fp64": f64["heights"],
+        "max_rel_res_mixed": mx["rel"], "max_rel_res_fp64": f64["rel"] if f64 else None,
+        "tile_rows_mixed": mx["heights"], "tile_rows_fp64": f64["heights"] if f64 else None,
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
@@ -390,7 +448,8 @@ def bench_transect(args):
     line = {
         "metric": METRIC, "value": round(args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-inner (fp16 line factors)/fp64-refinement",
         "data": "synthetic (ISW transect recipe)",
         "config": {"workload": "ISW transect 512x4x100 periodic y, Gaussian ridge", "cells": g.cells},
         "fp64_solves_per_s": round(args.steps / res["fp64"]["wall"], 3),
@@ -407,14 +466,18 @@ def bench_transect(args):
 
 
 def bench_serve(args):
-    """Serving: Poisson arrivals (rate 200/s), sizes log-uniform from 128x128x30 to 2048x1536x80,
-    10 time steps per request (RHS perturbed by 1 % N(0,1) per step, warm start from the resident
-    pressure), mixed precision to rel.res 1e-10, admission by free HBM, continuous batching.
-    The first `--requests` arrivals are served; sizes snap to `--pool` pre-assembled grids at
-    log-uniform quantiles (a request clones its grid's model: the operator is shared)."""
+    """Serving: Poisson arrivals, sizes log-uniform from 128x128x30 to 2048x1536x80, 10 time steps per
+    request (RHS perturbed by 1 % N(0,1) per step, warm start from the resident pressure), mixed precision
+    to rel.res 1e-10, admission by free HBM into cohorts of one size class that run concurrently on the
+    server's lanes. Sizes snap to a pool of `--pool` pre-assembled grids at t = 0, 1/(pool-1), ..., 1
+    (both ends of the range included; a request clones its grid's model, the operator is shared).
+    Two phases: the first `--requests` arrivals at `--rate` (200/s, far above capacity: a burst whose
+    latency is queueing), then `--requests2` arrivals at 60 % of the capacity the burst measured (a load
+    the GPU sustains). `value` is the burst's solves/s."""
     from orctm_amd.nh import NHServer
 
-    ts = [(q + 0.5) / args.pool for q in range(args.pool)]
+    P = max(args.pool, 2)
+    ts = [q / (P - 1) for q in range(P)]
     models = []
     t = time.perf_counter()
     for q, tq in enumerate(ts):
@@ -430,47 +493,83 @@ def bench_serve(args):
         models.append(m)
         log(f"[bench] serve: model {q}: {nx}x{ny}x{nz} = {nx * ny * nz} cells")
     log(f"[bench] serve: {len(models)} models assembled in {time.perf_counter() - t:.1f} s")
-    u = synth.uniform(synth.BASE_SEED, 505, 2 * args.requests)
-    gaps = -np.log(1.0 - u[: args.requests]) / args.rate
-    arrivals = np.cumsum(gaps)
-    sizes_t = u[args.requests:]
-    pick = [int(min(args.pool - 1, np.floor(x * args.pool))) for x in sizes_t]
     nsteps = 10
-    with NHServer() as srv:
+
+    def phase(srv, n, rate, stream):
+        u = synth.uniform(synth.BASE_SEED, stream, 2 * n)
+        arrivals = np.cumsum(-np.log(1.0 - u[:n]) / rate)
+        pick = [int(round(x * (P - 1))) for x in u[n:]]
         tickets = []
         t0 = time.perf_counter()
-        for k in range(args.requests):
+        for k in range(n):
             delay = arrivals[k] - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            tickets.append(srv.submit(models[pick[k]], nsteps, 0.01, 9000 + 100 * k))
+            tickets.append(srv.submit(models[pick[k]], nsteps, 0.01, 9000 + 100 * k + 7 * stream))
         stats = [srv.wait(tk) for tk in tickets]
         wall = time.perf_counter() - t0
+        bad = [s for s in stats if s["status"] != "OK"]
+        if bad:
+            raise RuntimeError(f"serve: {len(bad)} requests failed: {bad[:3]}")
+        lat = [s["t_done_ms"] - s["t_submit_ms"] for s in stats]
+        svc = [(s["t_done_ms"] - s["t_admit_ms"]) / nsteps for s in stats]
+        by_size = {}
+        for p_, l_, v_ in zip(pick, lat, svc):
+            by_size.setdefault(p_, ([], []))
+            by_size[p_][0].append(l_)
+            by_size[p_][1].append(v_)
+        classes = {}
+        for p_ in sorted(by_size):
+            l_, v_ = by_size[p_]
+            classes[f"{models[p_].grid.nx}x{models[p_].grid.ny}x{models[p_].grid.nz}"] = {
+                "requests": len(l_), "latency_ms_p50": round(pct(l_, 50), 1), "latency_ms_p99": round(pct(l_, 99), 1),
+                "service_ms_per_step_p50": round(pct(v_, 50), 2), "service_ms_per_step_p99": round(pct(v_, 99), 2)}
+        cells = sum(models[p_].grid.cells for p_ in pick)
+        cell_iters = sum(models[p_].grid.cells * s["inner_iters"] for p_, s in zip(pick, stats))
+        return dict(n=n, rate=rate, wall=wall, lat=lat, svc=svc, classes=classes, cells=cells, cell_iters=cell_iters,
+                    inner=sum(s["inner_iters"] for s in stats), rel=max(s["rel_res"] for s in stats))
+
+    with NHServer(lanes=args.lanes) as srv:
+        burst = phase(srv, args.requests, args.rate, 505)
+        log(f"[bench] serve burst: {args.requests} requests in {burst['wall']:.2f} s")
+        capacity = args.requests / burst["wall"]
+        steady = phase(srv, args.requests2, 0.6 * capacity, 506) if args.requests2 > 0 else None
         info = srv.info()
-    bad = [s for s in stats if s["status"] != "OK"]
-    if bad:
-        raise RuntimeError(f"serve: {len(bad)} requests failed: {bad[:3]}")
-    lat = [s["t_done_ms"] - s["t_submit_ms"] for s in stats]
-    svc = [(s["t_done_ms"] - s["t_admit_ms"]) / nsteps for s in stats]
-    cells = [models[p].grid.cells for p in pick]
     solves = nsteps * args.requests
+    # the inner matvec's bytes (36 B per cell, FP32) over every inner iteration of the burst, per second of
+    # its wall time: the whole service (admission, cohort set-up, the other kernels, idle lanes) counted
+    mv_gbs = 36.0 * burst["cell_iters"] / burst["wall"] / 1e9
+
+    def summary(ph):
+        return {"requests": ph["n"], "offered_per_s": round(ph["rate"], 3), "wall_s": round(ph["wall"], 3),
+                "solves_per_s": round(nsteps * ph["n"] / ph["wall"], 3),
+                "cell_solves_per_s": round(nsteps * ph["cells"] / ph["wall"], 1),
+                "latency_ms_p50": round(pct(ph["lat"], 50), 1), "latency_ms_p99": round(pct(ph["lat"], 99), 1),
+                "service_ms_per_step_p50": round(pct(ph["svc"], 50), 2),
+                "service_ms_per_step_p99": round(pct(ph["svc"], 99), 2), "by_size": ph["classes"]}
+
     line = {
-        "metric": METRIC, "value": round(solves / wall, 3), "unit": "solves/s", "n_gpus": 1,
-        "steps": solves, "warmup": 0, "ms_per_step": round(1e3 * wall / solves, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner/fp64-refinement",
+        "metric": METRIC, "value": round(solves / burst["wall"], 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": solves, "warmup": 0, "ms_per_step": round(1e3 * burst["wall"] / solves, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-inner (fp16 line factors)/fp64-refinement",
         "data": "synthetic (seeded generator, regional-tile recipe per grid)",
-        "config": {"workload": f"serving: Poisson arrivals {args.rate:g}/s, first {args.requests} requests x "
-                               f"{nsteps} warm-started steps, sizes log-uniform 128x128x30..2048x1536x80 on a "
-                               f"{args.pool}-grid pool, mixed to rel.res 1e-10, continuous batching",
+        "config": {"workload": f"serving: Poisson arrivals, first {args.requests} requests at {args.rate:g}/s, then "
+                               f"{args.requests2} at 60% of measured capacity; {nsteps} warm-started steps each, "
+                               f"sizes log-uniform 128x128x30..2048x1536x80 on a {P}-grid pool, mixed to rel.res "
+                               f"1e-10, {args.lanes} lanes of size-class cohorts",
                    "pool_cells": [m.grid.cells for m in models]},
         "requests": args.requests, "rounds": info["rounds"],
-        "latency_ms_p50": round(pct(lat, 50), 1), "latency_ms_p99": round(pct(lat, 99), 1),
-        "service_ms_per_step_p50": round(pct(svc, 50), 2), "service_ms_per_step_p99": round(pct(svc, 99), 2),
-        "cell_solves_per_s": round(nsteps * sum(cells) / wall, 1),
-        "mean_inner_iters_per_step": round(sum(s["inner_iters"] for s in stats) / solves, 2),
-        "max_rel_res": max(s["rel_res"] for s in stats),
-        "offered_load_note": "arrival rate far above service capacity: latency is queue-dominated",
-        "roofline": None, "cpu_baseline": None,
+        "latency_ms_p50": round(pct(burst["lat"], 50), 1), "latency_ms_p99": round(pct(burst["lat"], 99), 1),
+        "burst": summary(burst), "sustained": summary(steady) if steady else None,
+        "cell_solves_per_s": round(nsteps * burst["cells"] / burst["wall"], 1),
+        "mean_inner_iters_per_step": round(burst["inner"] / solves, 2),
+        "max_rel_res": max(burst["rel"], steady["rel"] if steady else 0.0),
+        "roofline": {"bound": "hbm", "kernel": "k_bwd<float> over the whole burst", "achieved": round(mv_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "the inner matvec's bytes (36 B per cell, FP32) of every inner iteration served, over "
+                             "the burst's wall time (admission, cohort set-up, the other kernels and idle lanes "
+                             "included): a lower bound of the service's streaming rate, not one kernel's"},
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     for m in models:

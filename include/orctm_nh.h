@@ -260,6 +260,13 @@ int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, doubl
  * solve's status is also what orctm_nh_divergence returns when it is not OK. */
 int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, int32_t* iterations,
                         double* rel_res, double* device_ms);
+/* With options.bottom_solve: ||K - K^T||_F / ||K||_F of the 2-D operator the bottom solve's CG runs on,
+ * K = -S A2 over the inner rows of the box (S = DLXP*DLYP; A2 from compute_matrix_2d, mo_petsc.f90:401-538),
+ * measured at create. It is 0 when DLXP is constant along j and DLYP along i; on other grids (spherical,
+ * curvilinear) K is not symmetric: CG with the true-residual check every 8 iterations still converges on
+ * the grids tested (defects up to 5e-3), and the solve reports ORCTM_NH_NOT_CONVERGED if it does not.
+ * No counterpart in the reference (its cg2d_ KSP is PETSc's default GMRES). */
+int orctm_nh_bottom_symmetry(orctm_nh_t* h, double* defect);
 
 /* ------------------------------------------------------------------------------------------
  * The barotropic free-surface solve of the hydrostatic core: ITPREP (ORCTM/src_orctm/itprep.f90:15-38,
@@ -329,20 +336,31 @@ int orctm_nh_batch_profile_read(orctm_nh_batch_t* batch, orctm_nh_profile_t* pro
  * Serving: a request loop with continuous batching. No counterpart in the reference (ORCTM solves
  * once per model time step on its own ranks). A request is a grid -- given as a "model" handle whose
  * operator and resident right-hand side (orctm_nh_set_rhs) define it -- and nsteps consecutive time
- * steps: between steps the right-hand side is perturbed on the device (orctm_nh_perturb_rhs,
- * amplitude `perturb`, stream seed + step) and the solve warm-starts from the pressure kept on the
- * device. The server thread admits queued requests (FIFO) while the slot count and the device-memory
- * budget allow (each admitted request holds a clone of its model: the operator is shared), advances
- * all admitted requests by one step per batched solve, and admits new arrivals between rounds.
- * A request larger than the whole budget is refused with ORCTM_NH_ERR_NOMEM. The model handle must
- * stay alive and unchanged until its requests have been admitted. */
+ * steps: between steps the right-hand side is perturbed on the device (the stream of
+ * orctm_nh_perturb_rhs, amplitude `perturb`, seed + step) and the solve warm-starts from the pressure
+ * kept on the device.
+ * The server runs `lanes` worker threads, each with its own stream, so that the work of several
+ * admitted requests overlaps on the device. A free lane admits the oldest queued request that fits
+ * the device-memory budget and the slot count (lane 0 takes only requests below 2^24 cells when there
+ * are two lanes or more, so that small requests are never all queued behind large ones), together
+ * with the queued requests of the same size class (floor(log2(cells))) and precision that also fit,
+ * up to max_cohort requests or 2^25 cells: a cohort. Each admitted request holds a clone of its model
+ * (the operator is shared). A cohort's requests advance together, one batched solve per time step
+ * (one launch per solver phase, per-request convergence masks); a request retires at the end of its
+ * last step, and the lane admits new requests as soon as its cohort is done, while other lanes keep
+ * running. A request larger than the whole budget is refused with ORCTM_NH_ERR_NOMEM. The model handle
+ * must stay alive and unchanged until its requests have been admitted. Each request's result is
+ * bit-identical alone and in any mix. */
 typedef struct orctm_nh_server orctm_nh_server_t;
 
 typedef struct orctm_nh_server_options {
-  int64_t hbm_budget;   /* bytes of device memory for admitted requests' state (<= 0: 90 % of free) */
+  int64_t hbm_budget;   /* bytes of device memory for admitted requests' state (<= 0: 90 % of what is free,
+                           counting the library's idle pooled memory as free) */
   int32_t max_active;   /* admitted requests at most (default 64) */
   int32_t device;       /* HIP device ordinal (default 0) */
-  int32_t reserved[8];
+  int32_t lanes;        /* worker lanes (streams) running cohorts concurrently (default 4, at most 16) */
+  int32_t max_cohort;   /* requests per cohort at most (default 16) */
+  int32_t reserved[6];
 } orctm_nh_server_options_t;
 
 typedef struct orctm_nh_request_stats {
@@ -367,6 +385,8 @@ int orctm_nh_server_submit(orctm_nh_server_t* server, orctm_nh_t* model, int32_t
 int orctm_nh_server_poll(orctm_nh_server_t* server, int64_t ticket, orctm_nh_request_stats_t* stats);
 int orctm_nh_server_wait(orctm_nh_server_t* server, int64_t ticket, orctm_nh_request_stats_t* stats);
 int orctm_nh_server_drain(orctm_nh_server_t* server);
+/* rounds: batched solves run so far (one per cohort and time step); bytes_in_use: state of the admitted
+ * requests; active: admitted requests not yet finished; queued: requests waiting for admission. */
 int orctm_nh_server_info(orctm_nh_server_t* server, int64_t* rounds, int64_t* bytes_in_use, int32_t* active,
                          int32_t* queued);
 /* Finishes every queued and admitted request, then stops the server thread. */

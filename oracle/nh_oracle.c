@@ -32,15 +32,24 @@
  *
  * Interior vectors use DMDA natural order: n = i + nx*(j + ny*k), i fastest (mo_petsc.f90:620-622).
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../include/orctm_nh.h"
 
 #define NC 7 /* coefficient planes: 0 centre, 1 west(i-1), 2 east(i+1), 3 north(j-1), 4 south(j+1),
                 5 up(k-1), 6 down(k+1) */
+
+/* Wall clock in milliseconds (the solvers report the time of their iterations apart from their set-up). */
+static double wall_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
 
 /* Fortran element accessors (1-based indices, column-major). */
 #define P2(a, I, J) ((a)[((size_t)(I)-1) + (size_t)IE * ((size_t)(J)-1)])
@@ -417,6 +426,7 @@ static double dot64(size_t n, const double* a, const double* b) {
 int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, const double* S, const double* b,
               double* p, int precision, double rtol, double rtol_inner, int maxit, int max_outer,
               orctm_nh_stats_t* st, const void* Cin) {
+  const double t_begin = wall_ms();
   const size_t N = (size_t)nx * ny * nz;
   orctm_nh_stats_t s;
   memset(&s, 0, sizeof s);
@@ -464,6 +474,7 @@ int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, con
   int inner_total = 0;
   int m = 0;
   double rel = 0.0;
+  const double t_iter = wall_ms(); /* set-up (inner operator, line factors, work arrays) done */
   for (;; ++m) {
     orc_matvec64(nx, ny, nz, cyc_x, cyc_y, A, p, r);
 #pragma omp parallel for schedule(static)
@@ -514,6 +525,8 @@ int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, con
   s.outer_iters = m;
   s.inner_iters = inner_total;
   s.rel_res = rel;
+  s.solve_ms = wall_ms() - t_iter;
+  s.total_ms = wall_ms() - t_begin;
   if (st) *st = s;
   free(r);
   free(C);
@@ -532,6 +545,7 @@ int orc_solve(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, con
 /* the true residual confirms it.                                                                */
 int orc_fgmres(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, const double* S, const double* b,
                double* x, int restart, double rtol, int maxit, orctm_nh_stats_t* st) {
+  const double t_begin = wall_ms();
   const size_t N = (size_t)nx * ny * nz;
   const int mr = restart > 0 ? restart : 30;
   orctm_nh_stats_t s;
@@ -563,6 +577,7 @@ int orc_fgmres(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, co
   double* y = (double*)malloc(sizeof(double) * mr);
   int its = 0, status = ORCTM_NH_NOT_CONVERGED;
   double rel = 0.0;
+  const double t_iter = wall_ms(); /* set-up (inner operator, line factors, Krylov basis) done */
   while (1) {
     orc_matvec64(nx, ny, nz, cyc_x, cyc_y, A, x, w);
 #pragma omp parallel for schedule(static)
@@ -642,6 +657,8 @@ int orc_fgmres(int nx, int ny, int nz, int cyc_x, int cyc_y, const double* A, co
   s.outer_iters = 0;
   s.inner_iters = its;
   s.rel_res = rel;
+  s.solve_ms = wall_ms() - t_iter;
+  s.total_ms = wall_ms() - t_begin;
   if (st) *st = s;
   free(C);
   free(f1);

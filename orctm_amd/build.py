@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["nh_kernels.hip", "nh_ocv.hip", "nh_sor.hip", "nh_bot.hip", "nh_api.hip", "nh_serve.cpp"]
+SOURCES = ["nh_kernels.hip", "nh_ocv.hip", "nh_sor.hip", "nh_bot.hip", "nh_pool.cpp", "nh_api.hip", "nh_serve.cpp"]
 OUT = os.path.join(HERE, "liborctm_nh.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ORCTM_ARCH", "gfx950")

@@ -25,6 +25,8 @@
 #include "nh_kernels.h"
 #include "nh_lock.h"
 #include "nh_ocv.h"
+#include "nh_internal.h"
+#include "nh_pool.h"
 
 using namespace onh;
 
@@ -58,7 +60,10 @@ struct Ctx {
   void* d_inner = nullptr;
   Outer* h_outer = nullptr;  // pinned
   void* h_inner = nullptr;   // pinned
+  unsigned long long* d_seeds = nullptr;  // per system: seed of the right-hand-side perturbation
+  unsigned long long* h_seeds = nullptr;  // pinned staging of d_seeds
   hipStream_t stream = nullptr;
+  bool own_stream = true;          // false: the stream belongs to the handle the context solves
   std::vector<Group> groups;       // inner-iteration tilings, one per tile height in use
   std::vector<int> heights;        // tile height of each system
   std::vector<int> tab;            // per system: 1 = ZW/ZP read from the table and window
@@ -95,9 +100,7 @@ struct OpData {
   std::vector<void*> allocs;
   size_t bytes = 0;
   ~OpData() {
-    if (allocs.empty()) return;
-    hipSetDevice(device);
-    for (void* p : allocs) hipFree(p);
+    for (void* p : allocs) pool_free(device, p);
   }
 };
 
@@ -109,6 +112,8 @@ struct orctm_nh {
   bool fp32 = true;
   bool guess_set = false;
   Sys sys{};  // host copy, pointers into device memory
+  hipStream_t stream = nullptr;    // the handle's own stream: its solver context and every copy use it
+  bool ctx_ready = false;          // the single-system context is built on first use (clones skip it)
   std::vector<void*> allocs;
   size_t bytes = 0;
   double* d_fort = nullptr;        // staging for a Fortran (IE,JE,KE) array
@@ -162,27 +167,29 @@ int map_err(hipError_t e) {
   return ORCTM_NH_ERR_HIP;
 }
 
+// Device memory of a handle comes from the library's pool and is zero-filled on the handle's own
+// stream, which every later use of it follows (no device-wide synchronisation, so handles can be
+// created and destroyed while other contexts run).
 hipError_t dalloc(orctm_nh_t* h, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = pool_alloc(h->opts.device, bytes, p);
   if (e != hipSuccess) return e;
   h->allocs.push_back(*p);
   h->bytes += bytes;
-  // the solver streams are non-blocking: finish the zero-fill before any of them can touch it
-  e = hipMemset(*p, 0, bytes);
-  return e == hipSuccess ? hipDeviceSynchronize() : e;
+  e = hipMemsetAsync(*p, 0, bytes, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
 }
 
 hipError_t opalloc(orctm_nh_t* h, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = pool_alloc(h->opts.device, bytes, p);
   if (e != hipSuccess) return e;
   h->op->allocs.push_back(*p);
   h->op->bytes += bytes;
-  e = hipMemset(*p, 0, bytes);
-  return e == hipSuccess ? hipDeviceSynchronize() : e;
+  e = hipMemsetAsync(*p, 0, bytes, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
 }
 
 // Per-handle state: right-hand side, solution, inner work vectors, block partials, ||b||.
@@ -222,11 +229,41 @@ hipError_t ensure_fort(orctm_nh_t* h) {
   return e;
 }
 
+hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<int>& forced,
+                     const std::vector<int>& zw_table, bool fp32, int check_every, int max_outer, int device,
+                     bool prof, hipStream_t stream);
+void ctx_destroy(Ctx& c);
+
+// The single-system context of a handle, built on its stream on first use: a clone that is only ever
+// solved inside a batch (the serving loop's requests) never builds one.
+int ensure_ctx(orctm_nh_t* h) {
+  if (h->ctx_ready) return ORCTM_NH_OK;
+  DEVLOCK;
+  hipError_t e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, {h->opts.zw_table}, h->fp32, h->opts.check_every,
+                           h->opts.max_outer, h->opts.device, h->opts.profile != 0, h->stream);
+  if (e != hipSuccess) {
+    hipStreamSynchronize(h->stream);
+    ctx_destroy(h->ctx);
+    return map_err(e);
+  }
+  h->ctx_ready = true;
+  return ORCTM_NH_OK;
+}
+
 void free_all(orctm_nh_t* h) {
   if (h->bot) bot_release(*h->bot);
   h->bot.reset();
-  for (void* p : h->allocs) hipFree(p);
+  for (void* p : h->allocs) pool_free(h->opts.device, p);
   h->allocs.clear();
+}
+
+// Drains the handle's stream, returns its memory to the pool and deletes it.
+void release(orctm_nh_t* h) {
+  if (h->stream) hipStreamSynchronize(h->stream);
+  ctx_destroy(h->ctx);
+  free_all(h);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
 }
 
 // Tiles (64 x h columns) of one system in natural (row-major) order.
@@ -251,23 +288,32 @@ std::vector<Tile> xcd_order(const std::vector<Tile>& nat) {
   return out;
 }
 
+// The caller has synchronised the context's stream (no queued work uses its blocks).
 void ctx_destroy(Ctx& c) {
   for (auto& kv : c.graphs) hipGraphExecDestroy(kv.second);
-  if (c.d_sys) hipFree(c.d_sys);
-  if (c.d_tiles) hipFree(c.d_tiles);
+  pool_free(c.device, c.d_sys);
+  pool_free(c.device, c.d_tiles);
   for (Group& g : c.groups) {
-    if (g.d_tiles) hipFree(g.d_tiles);
-    if (g.d_pairs) hipFree(g.d_pairs);
+    pool_free(c.device, g.d_tiles);
+    pool_free(c.device, g.d_pairs);
   }
-  if (c.d_outer) hipFree(c.d_outer);
-  if (c.d_inner) hipFree(c.d_inner);
-  if (c.h_outer) hipHostFree(c.h_outer);
-  if (c.h_inner) hipHostFree(c.h_inner);
+  pool_free(c.device, c.d_outer);
+  pool_free(c.device, c.d_inner);
+  pool_free_host(c.h_outer);
+  pool_free_host(c.h_inner);
+  pool_free(c.device, c.d_seeds);
+  pool_free_host(c.h_seeds);
   if (c.ev0) hipEventDestroy(c.ev0);
   if (c.ev1) hipEventDestroy(c.ev1);
   for (hipEvent_t e : c.kev) hipEventDestroy(e);
-  if (c.stream) hipStreamDestroy(c.stream);
+  if (c.stream && c.own_stream) hipStreamDestroy(c.stream);
   c = Ctx{};
+}
+
+// Host-to-device copy of a small host array on a stream (the host array may go out of scope once the
+// stream has been synchronised).
+hipError_t put_async(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
 }
 
 // One inner-iteration phase over every tile group, then the phase's per-system reduction. With e0/e1
@@ -333,8 +379,9 @@ struct TuneSet {
     for (size_t q = 0; q < members.size(); ++q) ts[q] = dev[members[q]];
   }
   ~TuneSet() {
-    if (d_ts) hipFree(d_ts);
-    if (d_in) hipFree(d_in);
+    hipStreamSynchronize(c.stream);
+    pool_free(c.device, d_ts);
+    pool_free(c.device, d_in);
   }
   hipError_t upload() {
     const size_t m = ts.size(), isz = inner_size(c.fp32);
@@ -350,11 +397,14 @@ struct TuneSet {
         in->rho = 1.0;
       }
     }
-    hipError_t e = hipMalloc(&d_ts, sizeof(Sys) * m);
-    if (e == hipSuccess) e = hipMalloc(&d_in, hin.size());
-    if (e == hipSuccess) e = hipMemcpy(d_in, hin.data(), hin.size(), hipMemcpyHostToDevice);
+    void* p = nullptr;
+    hipError_t e = pool_alloc(c.device, sizeof(Sys) * m, &p);
+    d_ts = static_cast<Sys*>(p);
+    if (e == hipSuccess) e = pool_alloc(c.device, hin.size(), &d_in);
+    if (e == hipSuccess) e = put_async(d_in, hin.data(), hin.size(), c.stream);
     for (size_t q = 0; q < m; ++q) ts[q].inner = static_cast<char*>(d_in) + isz * q;
-    if (e == hipSuccess) e = hipMemcpy(d_ts, ts.data(), sizeof(Sys) * m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = put_async(d_ts, ts.data(), sizeof(Sys) * m, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
     return e;
   }
   // one warm round, then 3 timed rounds of (matvec, column update), heights h[q] per system
@@ -368,18 +418,19 @@ struct TuneSet {
         if (h[q] == hc) order_into(nat, ts[q], (int)q, hc);
       if (nat.empty()) continue;
       std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
-      Tile *d = nullptr, *dp = nullptr;
-      e = hipMalloc(&d, sizeof(Tile) * ord.size());
+      void *d = nullptr, *dp = nullptr;
+      e = pool_alloc(c.device, sizeof(Tile) * ord.size(), &d);
       if (e != hipSuccess) break;
-      d_tiles.push_back(d);
-      e = hipMalloc(&dp, sizeof(Tile) * prs.size());
+      d_tiles.push_back(static_cast<Tile*>(d));
+      e = pool_alloc(c.device, sizeof(Tile) * std::max<size_t>(prs.size(), 1), &dp);
       if (e != hipSuccess) break;
-      d_pairs.push_back(dp);
+      d_pairs.push_back(static_cast<Tile*>(dp));
       nts.push_back((int)ord.size());
       nps.push_back((int)prs.size());
       hs.push_back(hc);
-      e = hipMemcpy(d, ord.data(), sizeof(Tile) * ord.size(), hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMemcpy(dp, prs.data(), sizeof(Tile) * prs.size(), hipMemcpyHostToDevice);
+      e = put_async(d, ord.data(), sizeof(Tile) * ord.size(), c.stream);
+      if (e == hipSuccess && !prs.empty()) e = put_async(dp, prs.data(), sizeof(Tile) * prs.size(), c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
       if (e != hipSuccess) break;
     }
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
@@ -392,8 +443,9 @@ struct TuneSet {
     if (e == hipSuccess) e = hipEventRecord(c.ev1, c.stream);
     if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, c.ev0, c.ev1);
-    for (Tile* d : d_tiles) hipFree(d);
-    for (Tile* d : d_pairs) hipFree(d);
+    hipStreamSynchronize(c.stream);
+    for (Tile* d : d_tiles) pool_free(c.device, d);
+    for (Tile* d : d_pairs) pool_free(c.device, d);
     return e;
   }
 };
@@ -495,15 +547,19 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       if (c.heights[q] == hc) order_into(nat, dev[q], q, hc);
     if (nat.empty()) continue;
     std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
-    Group g;
+    c.groups.push_back(Group{});
+    Group& g = c.groups.back();
     g.h = hc;
     g.ntiles = (int)ord.size();
     g.npairs = (int)prs.size();
-    HIPCHK(hipMalloc(&g.d_tiles, sizeof(Tile) * g.ntiles));
-    c.groups.push_back(g);
-    HIPCHK(hipMalloc(&c.groups.back().d_pairs, sizeof(Tile) * g.npairs));
-    HIPCHK(hipMemcpy(g.d_tiles, ord.data(), sizeof(Tile) * g.ntiles, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c.groups.back().d_pairs, prs.data(), sizeof(Tile) * g.npairs, hipMemcpyHostToDevice));
+    void* p = nullptr;
+    HIPCHK(pool_alloc(c.device, sizeof(Tile) * g.ntiles, &p));
+    g.d_tiles = static_cast<Tile*>(p);
+    HIPCHK(pool_alloc(c.device, sizeof(Tile) * std::max(g.npairs, 1), &p));
+    g.d_pairs = static_cast<Tile*>(p);
+    HIPCHK(put_async(g.d_tiles, ord.data(), sizeof(Tile) * g.ntiles, c.stream));
+    if (g.npairs) HIPCHK(put_async(g.d_pairs, prs.data(), sizeof(Tile) * g.npairs, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
   }
   return hipSuccess;
 }
@@ -515,9 +571,11 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
 // 12% and lowers that of a 512x4x100 transect by 14%, whose launches fill a few dozen CUs.
 constexpr long long kTabMinCells = 1LL << 20;
 
+// stream: the owning handle's stream for a single-system context, or nullptr for a context of its own
+// (a batch), which then creates one. Everything the build does is ordered on that stream.
 hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<int>& forced,
                      const std::vector<int>& zw_table, bool fp32, int check_every, int max_outer, int device,
-                     bool prof) {
+                     bool prof, hipStream_t stream) {
   c.nsys = (int)systems.size();
   c.prof = prof;
   c.iter_seen.assign(c.nsys, 0);
@@ -532,19 +590,31 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   for (int q = 0; q < c.nsys; ++q) order_into(nat, systems[q], q, TY);
   std::vector<Tile> ord = xcd_order(nat);
   c.ntiles = (int)ord.size();
-  HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  if (stream) {
+    c.stream = stream;
+    c.own_stream = false;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  }
   if (c.prof) {
     c.kev.assign(4 * (size_t)c.check_every, nullptr);
     for (hipEvent_t& e : c.kev) HIPCHK(hipEventCreate(&e));
   }
   HIPCHK(hipEventCreate(&c.ev0));
   HIPCHK(hipEventCreate(&c.ev1));
-  HIPCHK(hipMalloc(&c.d_outer, sizeof(Outer) * c.nsys));
-  HIPCHK(hipMemset(c.d_outer, 0, sizeof(Outer) * c.nsys));
-  HIPCHK(hipMalloc(&c.d_inner, inner_size(fp32) * c.nsys));
-  HIPCHK(hipMemset(c.d_inner, 0, inner_size(fp32) * c.nsys));
-  HIPCHK(hipHostMalloc((void**)&c.h_outer, sizeof(Outer) * c.nsys, hipHostMallocDefault));
-  HIPCHK(hipHostMalloc(&c.h_inner, inner_size(fp32) * c.nsys, hipHostMallocDefault));
+  void* p = nullptr;
+  HIPCHK(pool_alloc(device, sizeof(Outer) * c.nsys, &p));
+  c.d_outer = static_cast<Outer*>(p);
+  HIPCHK(hipMemsetAsync(c.d_outer, 0, sizeof(Outer) * c.nsys, c.stream));
+  HIPCHK(pool_alloc(device, inner_size(fp32) * c.nsys, &c.d_inner));
+  HIPCHK(hipMemsetAsync(c.d_inner, 0, inner_size(fp32) * c.nsys, c.stream));
+  HIPCHK(pool_alloc_host(sizeof(Outer) * c.nsys, &p));
+  c.h_outer = static_cast<Outer*>(p);
+  HIPCHK(pool_alloc_host(inner_size(fp32) * c.nsys, &c.h_inner));
+  HIPCHK(pool_alloc(device, sizeof(unsigned long long) * c.nsys, &p));
+  c.d_seeds = static_cast<unsigned long long*>(p);
+  HIPCHK(pool_alloc_host(sizeof(unsigned long long) * c.nsys, &p));
+  c.h_seeds = static_cast<unsigned long long*>(p);
   std::vector<Sys> dev = systems;
   c.tab.assign(c.nsys, 0);
   for (int q = 0; q < c.nsys; ++q) {
@@ -554,12 +624,13 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
     dev[q].tab = dev[q].tab && (mode == 1 || (mode == 0 && c.cells >= kTabMinCells)) ? 1 : 0;
     c.tab[q] = dev[q].tab;
   }
-  HIPCHK(hipMalloc(&c.d_sys, sizeof(Sys) * c.nsys));
-  HIPCHK(hipMemcpy(c.d_sys, dev.data(), sizeof(Sys) * c.nsys, hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&c.d_tiles, sizeof(Tile) * std::max(c.ntiles, 1)));
-  if (c.ntiles) HIPCHK(hipMemcpy(c.d_tiles, ord.data(), sizeof(Tile) * c.ntiles, hipMemcpyHostToDevice));
-  // the memsets above ran on the null stream; the tuning launches go to the context's stream
-  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(pool_alloc(device, sizeof(Sys) * c.nsys, &p));
+  c.d_sys = static_cast<Sys*>(p);
+  HIPCHK(put_async(c.d_sys, dev.data(), sizeof(Sys) * c.nsys, c.stream));
+  HIPCHK(pool_alloc(device, sizeof(Tile) * std::max(c.ntiles, 1), &p));
+  c.d_tiles = static_cast<Tile*>(p);
+  if (c.ntiles) HIPCHK(put_async(c.d_tiles, ord.data(), sizeof(Tile) * c.ntiles, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   return ctx_heights(c, dev, forced);
 }
 
@@ -709,6 +780,15 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
   return hipSuccess;
 }
 
+// b <- b (1 + amplitude xi) for every system of the context, system q with its own seed; ||b|| is
+// refreshed. Synchronous.
+hipError_t ctx_perturb(Ctx& c, double amplitude, const uint64_t* seeds) {
+  for (int q = 0; q < c.nsys; ++q) c.h_seeds[q] = seeds[q];
+  HIPCHK(hipMemcpyAsync(c.d_seeds, c.h_seeds, sizeof(unsigned long long) * c.nsys, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(launch_perturb(c.stream, c.d_sys, c.d_tiles, c.ntiles, c.nsys, amplitude, c.d_seeds));
+  return hipStreamSynchronize(c.stream);
+}
+
 int outer_status(const Outer& o) {
   switch (o.odone) {
     case CONVERGED: return ORCTM_NH_OK;
@@ -796,7 +876,7 @@ hipError_t bot_create(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->scal_host, 8 * sizeof(double), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  if (e == hipSuccess) e = bot_setup(h->ctx.stream, *b);
+  if (e == hipSuccess) e = bot_setup(h->stream, *b);
   if (e != hipSuccess) {
     bot_release(*b);
     return e;
@@ -877,6 +957,9 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   // temporary device copies of the Fortran inputs
   const size_t n2 = (size_t)g->ie * g->je, n3 = n2 * g->ke;
   const size_t nu = (size_t)(g->ie - g->i_start + 1) * g->je, nv = (size_t)g->ie * (g->je - g->j_start + 1);
+  // temporaries from the pool, all work ordered on the handle's stream
+  const int dev = h->opts.device;
+  hipStream_t st = h->stream;
   double *d_dlxp = nullptr, *d_dlyp = nullptr, *d_dlxu = nullptr, *d_dlyv = nullptr, *d_ddpo = nullptr,
          *d_weto = nullptr, *d_dzw = nullptr;
   Sys* d_sys1 = nullptr;
@@ -885,28 +968,37 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
   int* d_zbad = nullptr;
   int bad = 0, zbad = 1, rc = ORCTM_NH_OK;
   std::vector<Tile> tl = system_tiles(0, h->nx, h->ny, TY);
-  e = hipMalloc(&d_dlxp, n2 * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_dlyp, n2 * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_dlxu, nu * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_dlyv, nv * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_ddpo, n3 * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_weto, n3 * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_dzw, (size_t)g->ke * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_sys1, sizeof(Sys));
-  if (e == hipSuccess) e = hipMalloc(&d_tiles1, sizeof(Tile) * tl.size());
-  if (e == hipSuccess) e = hipMalloc(&d_bad, sizeof(int));
-  if (e == hipSuccess) e = hipMalloc(&d_zbad, sizeof(int));
-  if (e == hipSuccess) e = upload(d_dlxp, g->dlxp, n2 * 8);
-  if (e == hipSuccess) e = upload(d_dlyp, g->dlyp, n2 * 8);
-  if (e == hipSuccess) e = upload(d_dlxu, g->dlxu, nu * 8);
-  if (e == hipSuccess) e = upload(d_dlyv, g->dlyv, nv * 8);
-  if (e == hipSuccess) e = upload(d_ddpo, g->ddpo, n3 * 8);
-  if (e == hipSuccess) e = upload(d_weto, g->weto, n3 * 8);
-  if (e == hipSuccess) e = upload(d_dzw, g->dzw, (size_t)g->ke * 8);
-  if (e == hipSuccess) e = upload(d_sys1, &s, sizeof(Sys));
-  if (e == hipSuccess) e = upload(d_tiles1, tl.data(), sizeof(Tile) * tl.size());
-  if (e == hipSuccess) e = hipMemset(d_bad, 0, sizeof(int));
-  if (e == hipSuccess) e = hipMemset(d_zbad, 0, sizeof(int));
+  auto tmp = [&](auto*& ptr, size_t bytes) {
+    void* q = nullptr;
+    if (e == hipSuccess) e = pool_alloc(dev, bytes, &q);
+    ptr = static_cast<std::remove_reference_t<decltype(ptr)>>(q);
+  };
+  auto put = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  e = hipSuccess;
+  tmp(d_dlxp, n2 * 8);
+  tmp(d_dlyp, n2 * 8);
+  tmp(d_dlxu, nu * 8);
+  tmp(d_dlyv, nv * 8);
+  tmp(d_ddpo, n3 * 8);
+  tmp(d_weto, n3 * 8);
+  tmp(d_dzw, (size_t)g->ke * 8);
+  tmp(d_sys1, sizeof(Sys));
+  tmp(d_tiles1, sizeof(Tile) * tl.size());
+  tmp(d_bad, sizeof(int));
+  tmp(d_zbad, sizeof(int));
+  put(d_dlxp, g->dlxp, n2 * 8);
+  put(d_dlyp, g->dlyp, n2 * 8);
+  put(d_dlxu, g->dlxu, nu * 8);
+  put(d_dlyv, g->dlyv, nv * 8);
+  put(d_ddpo, g->ddpo, n3 * 8);
+  put(d_weto, g->weto, n3 * 8);
+  put(d_dzw, g->dzw, (size_t)g->ke * 8);
+  put(d_sys1, &s, sizeof(Sys));
+  put(d_tiles1, tl.data(), sizeof(Tile) * tl.size());
+  if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(int), st);
+  if (e == hipSuccess) e = hipMemsetAsync(d_zbad, 0, sizeof(int), st);
   if (e == hipSuccess) {
     FortranIn f{};
     f.IE = g->ie;
@@ -927,31 +1019,23 @@ int create_device(orctm_nh_t* h, const orctm_nh_grid_t* g) {
     f.ddpo = d_ddpo;
     f.weto = d_weto;
     f.dzw = d_dzw;
-    e = launch_assemble(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_bad, h->d_smax);
-    if (e == hipSuccess) e = launch_line_factor(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size());
-    if (e == hipSuccess) e = launch_zwindow(nullptr, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_zbad);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(&zbad, d_zbad, sizeof(int), hipMemcpyDeviceToHost);
+    e = launch_assemble(st, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_bad, h->d_smax);
+    if (e == hipSuccess) e = launch_line_factor(st, h->fp32, d_sys1, d_tiles1, (int)tl.size());
+    if (e == hipSuccess) e = launch_zwindow(st, h->fp32, d_sys1, d_tiles1, (int)tl.size(), f, d_zbad);
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&zbad, d_zbad, sizeof(int), hipMemcpyDeviceToHost, st);
   }
+  // every temporary is idle once the stream has drained (also after a failure part-way)
+  const hipError_t esync = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = esync;
   if (e != hipSuccess) rc = map_err(e);
   else if (bad) rc = ORCTM_NH_ERR_INVALID;
-  hipFree(d_dlxp);
-  hipFree(d_dlyp);
-  hipFree(d_dlxu);
-  hipFree(d_dlyv);
-  hipFree(d_ddpo);
-  hipFree(d_weto);
-  hipFree(d_dzw);
-  hipFree(d_sys1);
-  hipFree(d_tiles1);
-  hipFree(d_bad);
-  hipFree(d_zbad);
+  for (void* q : {(void*)d_dlxp, (void*)d_dlyp, (void*)d_dlxu, (void*)d_dlyv, (void*)d_ddpo, (void*)d_weto,
+                  (void*)d_dzw, (void*)d_sys1, (void*)d_tiles1, (void*)d_bad, (void*)d_zbad})
+    pool_free(dev, q);
   if (rc != ORCTM_NH_OK) return rc;
   s.tab = zbad ? 0 : 1;
-  e = ctx_build(h->ctx, {s}, {h->opts.tile_rows}, {h->opts.zw_table}, h->fp32, h->opts.check_every,
-                h->opts.max_outer, h->opts.device, h->opts.profile != 0);
-  return map_err(e);
+  return ensure_ctx(h);
 }
 
 // Padded device vector <-> natural-order host vector (elem bytes each).
@@ -971,7 +1055,7 @@ double now_ms() {
 int prepare_guess(orctm_nh_t* h) {
   // warm start: keep p64 (previous solution or set_guess); otherwise start from zero
   if (!h->opts.warm_start && !h->guess_set) {
-    hipError_t e = hipMemsetAsync(h->sys.p64, 0, (size_t)h->plane * h->nz * sizeof(double), h->ctx.stream);
+    hipError_t e = hipMemsetAsync(h->sys.p64, 0, (size_t)h->plane * h->nz * sizeof(double), h->stream);
     if (e != hipSuccess) return map_err(e);
   }
   h->guess_set = false;
@@ -1066,12 +1150,11 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   h->pitch = (h->nx + 31) / 32 * 32;
   h->plane = (long long)h->pitch * h->ny;
   h->fp32 = (o.precision == ORCTM_NH_MIXED);
-  int rc = create_device(h, grid);
+  int rc = map_err(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (rc == ORCTM_NH_OK) rc = create_device(h, grid);
   if (rc == ORCTM_NH_OK && o.bottom_solve) rc = map_err(bot_create(h, grid));
   if (rc != ORCTM_NH_OK) {
-    ctx_destroy(h->ctx);
-    free_all(h);
-    delete h;
+    release(h);
     return rc;
   }
   *handle = h;
@@ -1082,10 +1165,7 @@ int orctm_nh_destroy(orctm_nh_t* h) {
   if (!h) return ORCTM_NH_ERR_INVALID;
   DEVLOCK;
   hipSetDevice(h->opts.device);
-  if (h->ctx.stream) hipStreamSynchronize(h->ctx.stream);
-  ctx_destroy(h->ctx);
-  free_all(h);
-  delete h;
+  release(h);
   return ORCTM_NH_OK;
 }
 
@@ -1095,11 +1175,13 @@ int orctm_nh_set_rhs(orctm_nh_t* h, const double* divg) {
   DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
+  const int rc = ensure_ctx(h);
+  if (rc != ORCTM_NH_OK) return rc;
   hipError_t e = ensure_fort(h);
-  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, divg, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, divg, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess)
-    e = launch_rhs(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, h->d_fort, h->IE, h->JE);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+    e = launch_rhs(h->stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, h->d_fort, h->IE, h->JE);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return map_err(e);
 }
 
@@ -1109,11 +1191,13 @@ int orctm_nh_set_guess(orctm_nh_t* h, const double* pnh) {
   DEVLOCK;
   hipSetDevice(h->opts.device);
   const size_t n = (size_t)h->IE * h->JE * h->KE;
+  const int rc = ensure_ctx(h);
+  if (rc != ORCTM_NH_OK) return rc;
   hipError_t e = ensure_fort(h);
-  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, pnh, n * sizeof(double), hipMemcpyHostToDevice, h->ctx.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->d_fort, pnh, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess)
-    e = launch_guess(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+    e = launch_guess(h->stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, h->d_fort, h->IE, h->JE);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess) h->guess_set = true;
   return map_err(e);
 }
@@ -1123,7 +1207,8 @@ int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
   std::lock_guard<std::mutex> lk(h->mu);
   const double t0 = now_ms();
   hipSetDevice(h->opts.device);
-  int rc = prepare_guess(h);
+  int rc = ensure_ctx(h);
+  if (rc == ORCTM_NH_OK) rc = prepare_guess(h);
   if (rc != ORCTM_NH_OK) return rc;
   float ms = 0.f;
   hipError_t e = ctx_solve(h->ctx, &ms);
@@ -1170,6 +1255,10 @@ int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
   hipSetDevice(h->opts.device);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
   const size_t bytes = (size_t)h->plane * h->nz * sizeof(double);
   double *dx = nullptr, *dy = nullptr;
   hipError_t e = hipMalloc(&dx, bytes);
@@ -1177,8 +1266,8 @@ int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
   if (e == hipSuccess) e = hipMemset(dx, 0, bytes);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = to_device_vec(h, dx, x, sizeof(double));
-  if (e == hipSuccess) e = launch_apply64(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dx, dy);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  if (e == hipSuccess) e = launch_apply64(h->stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dx, dy);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess) e = to_host_vec(h, y, dy, sizeof(double));
   hipFree(dx);
   hipFree(dy);
@@ -1190,13 +1279,17 @@ int orctm_nh_get_operator(orctm_nh_t* h, double* coef7) {
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
   hipSetDevice(h->opts.device);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
   const size_t NP = (size_t)h->plane * h->nz, N = (size_t)h->nx * h->ny * h->nz;
   double* dA = nullptr;
   hipError_t e = hipMalloc(&dA, 7 * NP * sizeof(double));
   if (e == hipSuccess) e = hipMemset(dA, 0, 7 * NP * sizeof(double));
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = launch_materialize(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dA);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
+  if (e == hipSuccess) e = launch_materialize(h->stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dA);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   for (int q = 0; q < 7 && e == hipSuccess; ++q) e = to_host_vec(h, coef7 + q * N, dA + q * NP, sizeof(double));
   hipFree(dA);
   return map_err(e);
@@ -1221,6 +1314,10 @@ int orctm_nh_inner_apply(orctm_nh_t* h, const void* x, void* y) {
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
   hipSetDevice(h->opts.device);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
   // x goes where the forward elimination leaves w; the matvec kernel in raw mode takes it as z
@@ -1253,6 +1350,10 @@ int orctm_nh_inner_precond(orctm_nh_t* h, const void* r, void* z, double* rho) {
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
   hipSetDevice(h->opts.device);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
   const size_t es = h->fp32 ? 4 : 8;
   Ctx& c = h->ctx;
   // forward elimination (init kernel) then back substitution inside the matvec kernel, whose first
@@ -1314,21 +1415,18 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out) {
   h->open_s = src->open_s;
   h->sys = src->sys;  // operator pointers shared; state pointers replaced below
   h->d_smax = src->d_smax;
-  int rc = alloc_state(h);
-  hipError_t e = hipSuccess;
+  int rc = map_err(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (rc == ORCTM_NH_OK) rc = alloc_state(h);
   if (rc == ORCTM_NH_OK) {
+    // src's right-hand side was completed on src's stream before its entry point returned
     const size_t NP = (size_t)h->plane * h->nz;
-    e = hipMemcpy(h->sys.b, src->sys.b, NP * sizeof(double), hipMemcpyDeviceToDevice);
-    if (e == hipSuccess) e = hipMemcpy(h->sys.nbp, src->sys.nbp, sizeof(double), hipMemcpyDeviceToDevice);
-    if (e == hipSuccess)
-      e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, {h->opts.zw_table}, h->fp32, h->opts.check_every,
-                    h->opts.max_outer, h->opts.device, h->opts.profile != 0);
+    hipError_t e = hipMemcpyAsync(h->sys.b, src->sys.b, NP * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->sys.nbp, src->sys.nbp, sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     rc = map_err(e);
   }
   if (rc != ORCTM_NH_OK) {
-    ctx_destroy(h->ctx);
-    free_all(h);
-    delete h;
+    release(h);
     return rc;
   }
   *out = h;
@@ -1340,9 +1438,11 @@ int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed) {
   std::lock_guard<std::mutex> lk(h->mu);
   DEVLOCK;
   hipSetDevice(h->opts.device);
-  hipError_t e = launch_perturb(h->ctx.stream, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, 1, amplitude, seed);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->ctx.stream);
-  return map_err(e);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
+  return map_err(ctx_perturb(h->ctx, amplitude, &seed));
 }
 
 int orctm_nh_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps) {
@@ -1458,9 +1558,13 @@ int orctm_nh_divergence(orctm_nh_t* h, const double* uoo, const double* uoo_1, c
   DEVLOCK;
   if (!h->ocv_ready) return ORCTM_NH_ERR_INVALID;
   hipSetDevice(h->opts.device);
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
   const size_t IE = h->IE, JE = h->JE, KE = h->KE;
   const size_t u3 = (IE - h->i_start + 1) * JE * KE, v3 = IE * (JE - h->j_start + 1) * KE, w3 = IE * JE * (KE + 1);
-  hipStream_t st = h->ctx.stream;
+  hipStream_t st = h->stream;
   hipError_t e = hipMemcpyAsync(h->d_uoo, uoo, u3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_uoo_1, uoo_1, u3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_voe, voe, v3 * 8, hipMemcpyHostToDevice, st);
@@ -1513,6 +1617,14 @@ int orctm_nh_get_bottom(orctm_nh_t* h, double* pnh_bot, double* divg_bot_surf, i
   return map_err(e);
 }
 
+int orctm_nh_bottom_symmetry(orctm_nh_t* h, double* defect) {
+  if (!h || !defect) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  if (!h->bot) return ORCTM_NH_ERR_INVALID;
+  *defect = h->bot->sym_defect;
+  return ORCTM_NH_OK;
+}
+
 int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, double* wno) {
   if (!h || !pnh || !uoo || !voe || !wno) return ORCTM_NH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(h->mu);
@@ -1522,7 +1634,7 @@ int orctm_nh_project(orctm_nh_t* h, double* pnh, double* uoo, double* voe, doubl
   const size_t IE = h->IE, JE = h->JE, KE = h->KE;
   const size_t p3 = IE * JE * KE, u3 = (IE - h->i_start + 1) * JE * KE, v3 = IE * (JE - h->j_start + 1) * KE,
                w3 = IE * JE * (KE + 1);
-  hipStream_t st = h->ctx.stream;
+  hipStream_t st = h->stream;
   hipError_t e = hipMemcpyAsync(h->d_fort, pnh, p3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_uoo, uoo, u3 * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(h->d_voe, voe, v3 * 8, hipMemcpyHostToDevice, st);
@@ -1562,8 +1674,9 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
   if (!b) return ORCTM_NH_ERR_NOMEM;
   b->members.assign(systems, systems + n);
   hipSetDevice(dev);
-  hipError_t e = ctx_build(b->ctx, ss, forced, zwt, fp32, check, max_outer, dev, prof);
+  hipError_t e = ctx_build(b->ctx, ss, forced, zwt, fp32, check, max_outer, dev, prof, nullptr);
   if (e != hipSuccess) {
+    if (b->ctx.stream) hipStreamSynchronize(b->ctx.stream);
     ctx_destroy(b->ctx);
     delete b;
     return map_err(e);
@@ -1596,6 +1709,18 @@ int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
   return rc;
 }
 
+}  // extern "C"
+
+int onh::handle_precision(const orctm_nh_t* h) { return h ? (h->fp32 ? ORCTM_NH_MIXED : ORCTM_NH_FP64) : -1; }
+
+int onh::batch_perturb(orctm_nh_batch_t* b, double amplitude, const uint64_t* seeds) {
+  if (!b || !seeds) return ORCTM_NH_ERR_INVALID;
+  hipSetDevice(b->ctx.device);
+  return map_err(ctx_perturb(b->ctx, amplitude, seeds));
+}
+
+extern "C" {
+
 static int profile_read(Ctx& c, orctm_nh_profile_t* p, int32_t reset) {
   if (!p) return ORCTM_NH_ERR_INVALID;
   memset(p, 0, sizeof(*p));
@@ -1624,12 +1749,11 @@ int orctm_nh_set_profile(orctm_nh_t* h, int32_t on) {
   const int want = on ? 1 : 0;
   if ((h->opts.profile != 0) == (want != 0)) return ORCTM_NH_OK;
   hipSetDevice(h->opts.device);
-  if (h->ctx.stream) hipStreamSynchronize(h->ctx.stream);
+  if (h->stream) hipStreamSynchronize(h->stream);
   ctx_destroy(h->ctx);
+  h->ctx_ready = false;
   h->opts.profile = want;
-  hipError_t e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, {h->opts.zw_table}, h->fp32,
-                           h->opts.check_every, h->opts.max_outer, h->opts.device, want != 0);
-  return map_err(e);
+  return ensure_ctx(h);
 }
 
 int orctm_nh_batch_profile_read(orctm_nh_batch_t* b, orctm_nh_profile_t* p, int32_t reset) {

@@ -39,7 +39,8 @@ struct BotState {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the last bot_step
   int iters = -1;                  // iterations of the last solve, -1 before the first
   double rel_res = 0.0;
-  int status = 0;
+  int status = 0;                  // of the last solve; a solve after one that was not OK starts cold
+  double sym_defect = 0.0;         // ||K - K^T||_F / ||K||_F of the box operator K = -S A2 (bot_setup)
 };
 
 constexpr int kBotChunk = 8;   // PCG iterations per captured graph, between residual checks

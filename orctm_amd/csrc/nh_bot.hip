@@ -884,6 +884,24 @@ hipError_t mg_build(hipStream_t st, BotState& b) {
         if (act(i, j - 1)) L0.k[3 * n + c] = -S[c] * coef[3 * n + c];
         if (act(i, j + 1)) L0.k[4 * n + c] = -S[c] * coef[4 * n + c];
       }
+    // ||K - K^T||_F / ||K||_F over the inner rows: 0 when DLXP is constant along j and DLYP along i;
+    // on other grids (spherical, curvilinear) K is not symmetric and CG is applied to it regardless
+    double num = 0.0, den = 0.0;
+    for (int j = 1; j < ny - 1; ++j)
+      for (int i = 1; i < nx - 1; ++i) {
+        const size_t c = (size_t)j * nx + i;
+        for (int p = 0; p < 5; ++p) den += L0.k[p * n + c] * L0.k[p * n + c];
+        // west against the west neighbour's east, north against the north neighbour's south
+        if (act(i - 1, j)) {
+          const double d = L0.k[n + c] - L0.k[2 * n + c - 1];
+          num += 2.0 * d * d;
+        }
+        if (act(i, j - 1)) {
+          const double d = L0.k[3 * n + c] - L0.k[4 * n + c - nx];
+          num += 2.0 * d * d;
+        }
+      }
+    b.sym_defect = den > 0.0 ? std::sqrt(num / den) : 0.0;
     lv.push_back(std::move(L0));
   }
   auto unknowns = [](const HLevel& L) { return (long long)(L.a - 2 * L.o) * (L.b - 2 * L.o); };
@@ -1010,8 +1028,10 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
   hipLaunchKernelGGL(k_bot_rhs, dim3(nblocks(n)), dim3(BB), 0, st, g, a, (const double*)b.botgu,
                      (const double*)b.botgv, (const double*)b.botgw, divg, b.dbs, b.b);
   BCHK(hipGetLastError());
-  // the first solve starts from zero, later ones from the previous solution (mo_petsc.f90:165)
-  const int cold = b.iters < 0 ? 1 : 0;
+  // the first solve starts from zero, later ones from the previous solution (mo_petsc.f90:165); after a
+  // solve that did not converge (a non-finite right-hand side or iterate among the causes) the next one
+  // starts from zero again, so that one bad step does not poison every later one
+  const int cold = (b.iters < 0 || b.status != ORCTM_NH_OK) ? 1 : 0;
   hipLaunchKernelGGL(k2_start, dim3(nblocks(n)), dim3(BB), 0, st, g, (const double*)b.coef, (const double*)b.b,
                      b.x, b.d0, b.d1, cold);
   BCHK(hipGetLastError());
@@ -1030,7 +1050,8 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
   const double bb = b.scal_host[S_BB], tol2 = rtol * rtol * bb;
   double rr = b.scal_host[S_RR];
   int iters = 0, status = 0;
-  while (has_inner && rr > tol2 && iters < maxit) {
+  if (!std::isfinite(bb) || !std::isfinite(rr)) status = ORCTM_NH_ERR_NONFINITE;
+  while (status == 0 && has_inner && rr > tol2 && iters < maxit) {
     if (!b.chunk) {
       hipGraph_t graph = nullptr;
       BCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1050,6 +1071,10 @@ hipError_t bot_step(hipStream_t st, BotState& b, const OcvArrays& a, const doubl
     BCHK(hipGetLastError());
     BCHK(read_scal(st, b));
     rr = b.scal_host[S_RR];
+    if (!std::isfinite(rr)) {
+      status = ORCTM_NH_ERR_NONFINITE;
+      break;
+    }
     if (b.scal_host[S_FLAG] != 0.0 && !(rr <= tol2)) {
       status = ORCTM_NH_BREAKDOWN;
       break;

@@ -1,0 +1,18 @@
+// nh_internal.h -- library-internal entry points shared by the solver (nh_api.hip) and the serving
+// loop (nh_serve.cpp); not part of the C ABI.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/orctm_nh.h"
+
+namespace onh {
+
+// Perturbs every member's resident right-hand side, member q with seeds[q] (the same stream of
+// numbers as orctm_nh_perturb_rhs with that seed), in one launch over the batch; synchronous.
+int batch_perturb(orctm_nh_batch_t* b, double amplitude, const uint64_t* seeds);
+
+// The precision mode of a handle (ORCTM_NH_MIXED or ORCTM_NH_FP64); -1 for a null handle.
+int handle_precision(const orctm_nh_t* h);
+
+}  // namespace onh

@@ -26,9 +26,9 @@ hipError_t launch_zwindow(hipStream_t st, bool fp32, const Sys* d_sys, const Til
 // b = DIVG*WETO from a device copy of DIVG(IE,JE,KE); ||b|| into Outer.nb.
 hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
                       const double* d_divg, int IE, int JE);
-// b <- b (1 + amplitude xi), xi ~ N(0,1) from (seed, cell); ||b|| refreshed.
+// b <- b (1 + amplitude xi), xi ~ N(0,1) from (d_seeds[system], cell); ||b|| refreshed.
 hipError_t launch_perturb(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
-                          double amplitude, unsigned long long seed);
+                          double amplitude, const unsigned long long* d_seeds);
 // Interior of a device Fortran (IE,JE,KE) array -> p64 (initial guess).
 hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_pnh, int IE,
                         int JE);

@@ -555,7 +555,8 @@ __global__ __launch_bounds__(NT) void k_rhs(const Sys* __restrict__ systems, con
 }
 
 // b <- b * (1 + amplitude * xi), xi ~ N(0,1) per cell from a counter-based SplitMix64 stream
-// keyed by (seed, cell index); land cells stay 0. Synthetic time-step forcing for serving loads.
+// keyed by (seed of the system, cell index); land cells stay 0. Synthetic time-step forcing for
+// serving loads. seeds[q] is system q's seed, so one launch perturbs every system of a batch.
 __device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -564,7 +565,7 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
 }
 
 __global__ __launch_bounds__(NT) void k_perturb(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                                double amplitude, unsigned long long seed) {
+                                                double amplitude, const unsigned long long* __restrict__ seeds) {
   const Tile t = tiles[blockIdx.x];
   const Sys& s = systems[t.sys];
   const int i = t.bx * TX + (threadIdx.x & (TX - 1)), j = t.by * TY + (threadIdx.x >> 6);
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(NT) void k_perturb(const Sys* __restrict__ systems,
   if (i < s.nx && j < s.ny) {
     gptr<double> b = G<double>(s.b);
     const size_t col = (size_t)j * s.pitch + i;
-    const unsigned long long key = splitmix64(seed);
+    const unsigned long long key = splitmix64(seeds[t.sys]);
     for (int k = 0; k < s.nz; ++k) {
       const size_t idx = col + (size_t)k * s.plane;
       const unsigned long long n = ((unsigned long long)k * s.ny + j) * s.nx + i;
@@ -1643,8 +1644,8 @@ hipError_t launch_rhs(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int
 }
 
 hipError_t launch_perturb(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys,
-                          double amplitude, unsigned long long seed) {
-  PHASE(k_perturb, (launch_reduce<double, RED_NB>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, amplitude, seed);
+                          double amplitude, const unsigned long long* d_seeds) {
+  PHASE(k_perturb, (launch_reduce<double, RED_NB>(st, d_sys, nsys)), ntiles, st, d_sys, d_tiles, amplitude, d_seeds);
 }
 
 hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, const double* d_pnh, int IE,

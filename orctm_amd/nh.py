@@ -140,6 +140,12 @@ class NHSolver:
                                               ctypes.byref(ms)), "orctm_nh_get_bottom")
         return {"pnh_bot": p, "divg_bot_surf": d, "iterations": it.value, "rel_res": rr.value, "device_ms": ms.value}
 
+    def bottom_symmetry(self) -> float:
+        """With bottom_solve: ||K - K^T||_F / ||K||_F of the bottom solve's 2-D operator (orctm_nh_bottom_symmetry)."""
+        d = ctypes.c_double()
+        self._rc(self.lib.orctm_nh_bottom_symmetry(self.h, ctypes.byref(d)), "orctm_nh_bottom_symmetry")
+        return d.value
+
     def project(self, pnh, uoo, voe, wno) -> None:
         """PNH halo, U/V pressure correction and WNO by continuity (ocvtot_nh.f90:151-230), in place."""
         for a, kind, n in zip((pnh, uoo, voe, wno), ("p3", "u3", "v3", "w3"), ("pnh", "uoo", "voe", "wno")):
@@ -293,15 +299,23 @@ class NHBatch:
 
 
 class NHServer:
-    """Continuous-batching request loop (orctm_nh_server_*): requests = model solver + time steps."""
+    """Continuous-batching request loop (orctm_nh_server_*): requests = model solver + time steps,
+    admitted by free device memory into cohorts of one size class that run concurrently on lanes."""
 
-    def __init__(self, hbm_budget: int = 0, max_active: int = 64, device: int = 0):
+    def __init__(self, hbm_budget: int = 0, max_active: int = 64, device: int = 0, lanes: int = 0,
+                 max_cohort: int = 0):
+        """lanes: worker lanes running cohorts concurrently (0 = the default, 4); max_cohort: requests
+        per cohort at most (0 = the default, 16)."""
         self.lib = abi.load()
         o = abi.ServerOptionsC()
         self.lib.orctm_nh_server_default_options(ctypes.byref(o))
         o.hbm_budget = hbm_budget
         o.max_active = max_active
         o.device = device
+        if lanes:
+            o.lanes = lanes
+        if max_cohort:
+            o.max_cohort = max_cohort
         srv = ctypes.c_void_p()
         rc = self.lib.orctm_nh_server_create(ctypes.byref(o), ctypes.byref(srv))
         if rc != abi.OK:

@@ -106,6 +106,8 @@ class GridSpec:
     sigma_u: float = 0.2
     seed: int = BASE_SEED
     dx_profile: Optional[np.ndarray] = None  # optional DLXP(i) along x for the interior, length nx
+    lat_deg: Optional[tuple] = None  # (lat0, lat1): a spherical grid, DLXP scaled by cos(latitude), the
+    #                                  latitude linear in j from lat0 at J=1 to lat1 at J=JE
 
     @property
     def cells(self) -> int:
@@ -241,6 +243,9 @@ def make_grid(spec: GridSpec) -> Grid:
     else:
         dxi = np.full(IE, spec.dx)
     dlxp = np.asfortranarray(np.repeat(dxi[:, None], JE, axis=1))
+    if spec.lat_deg is not None:
+        lat = np.radians(np.linspace(spec.lat_deg[0], spec.lat_deg[1], JE))
+        dlxp = np.asfortranarray(dlxp * np.cos(lat)[None, :])
     dlyp = np.full((IE, JE), spec.dy, order="F")
     dlxu = np.empty((IE + 1, JE), order="F")  # I = 0..IE
     dlxu[1:IE, :] = 0.5 * (dlxp[:-1, :] + dlxp[1:, :])

@@ -155,3 +155,68 @@ def test_get_bottom_needs_the_option():
         with pytest.raises(Exception) as ei:
             s.get_bottom()
         assert getattr(ei.value, "code", None) == abi.ERR_INVALID
+
+
+def _k_asymmetry(g, c):
+    nx, ny = g.nx, g.ny
+    A = oracle.op2d_scipy(g, c).tocsr()
+    S = (g.dlxp[1:-1, 1:-1] * g.dlyp[1:-1, 1:-1]).ravel(order="F")
+    i, j = np.arange(nx * ny) % nx, np.arange(nx * ny) // nx
+    inner = np.flatnonzero((i > 0) & (i < nx - 1) & (j > 0) & (j < ny - 1))
+    K = A.multiply(-S[:, None]).tocsr()[inner][:, inner]
+    return spl.norm(K - K.T) / spl.norm(K)
+
+
+@pytest.mark.parametrize("lat", [(30.0, 75.0), (-70.0, 70.0)])
+def test_bottom_solve_on_a_spherical_grid(lat):
+    """DLXP scaled by cos(latitude) varies along j, so the CG's operator K = -S A2 is not symmetric: the
+    device reports the defect ||K - K^T|| / ||K|| that a scipy restatement measures, and its PNH_bot
+    still reaches rel.res 1e-10 and the direct solve's answer (DIVG_bot_surf bit-exact as everywhere)."""
+    print(f"[bot] spherical {lat}", flush=True)
+    g, _ = synth.build(synth.GridSpec(96, 80, 12, smooth=4.0, seed=31, lat_deg=lat))
+    v = synth.make_velocity(g)
+    d = oracle.divergence(g, v)
+    dbs, *_ = oracle.bottom_rhs(g, v, d)
+    with NHSolver(g, options(precision=abi.FP64, bottom_solve=True)) as s:
+        s.attach_vgrid(v)
+        out = step(s, v)
+        defect = s.bottom_symmetry()
+    c = oracle.op2d_coef(g)
+    asym = _k_asymmetry(g, c)
+    assert 1e-3 < asym and abs(defect - asym) <= 1e-6 * asym, (defect, asym)
+    assert bits_equal(out["divg_bot_surf"][1:-1, 1:-1], dbs[1:-1, 1:-1])
+    b = dbs[1:-1, 1:-1].ravel(order="F")
+    x = out["pnh_bot"][1:-1, 1:-1].ravel(order="F")
+    xd = spl.spsolve(oracle.op2d_scipy(g, c).tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    assert out["rel_res"] <= 1e-10
+
+
+def test_bottom_symmetry_is_zero_on_a_planar_grid():
+    g, v = case("stretched_dx")
+    with NHSolver(g, options(precision=abi.MIXED, bottom_solve=True)) as s:
+        assert s.bottom_symmetry() <= 1e-15
+
+
+def test_bottom_solve_recovers_after_a_nonfinite_step():
+    """A NaN velocity makes the 2-D solve's right-hand side non-finite: the step reports
+    ERR_NONFINITE, and the next clean step starts from zero (not from the poisoned iterate) and
+    converges, with the same bits as a fresh handle's first solve of that step."""
+    print("[bot] non-finite step", flush=True)
+    g, v = case("regional")
+    bad = copy.copy(v)
+    bad.uoo = v.uoo.copy(order="F")
+    i, j = np.argwhere(v.amsuo[1:-1, 1:-1, 0] == 1.0)[0] + 1
+    bad.uoo[i, j, 0] = np.nan  # top layer: DIVG(:,:,1) enters DIVG_bot_surf
+    with NHSolver(g, options(precision=abi.FP64, bottom_solve=True)) as s:
+        s.attach_vgrid(v)
+        with pytest.raises(Exception) as ei:
+            step(s, bad)
+        assert getattr(ei.value, "code", None) == abi.ERR_NONFINITE
+        rec = step(s, v)
+    with NHSolver(g, options(precision=abi.FP64, bottom_solve=True)) as s:
+        s.attach_vgrid(v)
+        fresh = step(s, v)
+    assert np.all(np.isfinite(rec["pnh_bot"])) and rec["rel_res"] <= 1e-10
+    assert rec["iterations"] == fresh["iterations"]
+    assert bits_equal(rec["pnh_bot"], fresh["pnh_bot"])
