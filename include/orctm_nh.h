@@ -162,6 +162,15 @@ int orctm_nh_get_solution(orctm_nh_t* h, double* pnh);              /* mo_petsc.
  * (i fastest, then j, then k) -- the order of res_array in mo_petsc.f90:222-227. */
 int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y);
 
+/* orctm_nh_apply without any device allocation: no counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same y = A x, the same bits. x and y are staged in a fixed device array
+ * of the library (128 MiB on each GPU it runs on, held until the process ends), so a system may have
+ * at most 8388608 padded cells ((IE-2) rounded up to 32, times (JE-2), times KE); larger ones return
+ * ORCTM_NH_ERR_INVALID before any device work. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (from several threads) race and must be serialised by the caller. */
+int orctm_nh_apply_noalloc(orctm_nh_t* h, const double* x, double* y);
+
 /* Reads back the assembled reference operator: 7 planes of (IE-2)*(JE-2)*KE doubles, in the order
  * centre, west(i-1), east(i+1), north(j-1), south(j+1), up(k-1), down(k+1); absent couplings are 0.
  * These are the MatSetValuesStencil values of mo_petsc.f90:623-765. */

@@ -111,7 +111,7 @@ EXPORTS = [
     "orctm_nh_server_poll", "orctm_nh_server_wait", "orctm_nh_server_drain", "orctm_nh_server_info",
     "orctm_nh_server_destroy", "orctm_nh_attach_vgrid", "orctm_nh_divergence", "orctm_nh_project",
     "orctm_nh_set_profile", "orctm_nh_sor_create", "orctm_nh_troneu", "orctm_nh_sor_info",
-    "orctm_nh_sor_destroy", "orctm_nh_get_bottom", "orctm_nh_bottom_symmetry",
+    "orctm_nh_sor_destroy", "orctm_nh_get_bottom", "orctm_nh_apply_noalloc", "orctm_nh_bottom_symmetry",
 ]
 
 _lib = None
@@ -143,6 +143,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.orctm_nh_solve_resident.argtypes = [vp, ctypes.POINTER(StatsC)]
     lib.orctm_nh_get_solution.argtypes = [vp, _dp]
     lib.orctm_nh_apply.argtypes = [vp, _dp, _dp]
+    lib.orctm_nh_apply_noalloc.argtypes = [vp, _dp, _dp]
     lib.orctm_nh_bottom_symmetry.argtypes = [vp, _dp]
     lib.orctm_nh_get_operator.argtypes = [vp, _dp]
     lib.orctm_nh_get_dims.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int32)] * 3

@@ -1052,6 +1052,32 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Staging of orctm_nh_apply_noalloc, for callers that cannot allocate device memory: x and y of at
+// most kScratchCells padded cells each, in one file-scope device array. Every GPU the library runs
+// on holds its own copy (128 MiB) from the loading of the code object until the process ends. The
+// array's name is a host-side stand-in: host code uses only the address hipGetSymbolAddress returns
+// for the current device, looked up once per device and kept in g_scratch_ptr.
+constexpr size_t kScratchCells = size_t(1) << 23;
+static __device__ __align__(256) double scratch[2 * kScratchCells];
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+hipError_t scratch_address(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    DEVLOCK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 int prepare_guess(orctm_nh_t* h) {
   // warm start: keep p64 (previous solution or set_guess); otherwise start from zero
   if (!h->opts.warm_start && !h->guess_set) {
@@ -1271,6 +1297,40 @@ int orctm_nh_apply(orctm_nh_t* h, const double* x, double* y) {
   if (e == hipSuccess) e = to_host_vec(h, y, dy, sizeof(double));
   hipFree(dx);
   hipFree(dy);
+  return map_err(e);
+}
+
+int orctm_nh_apply_noalloc(orctm_nh_t* h, const double* x, double* y) {
+  if (!h || !x || !y) return ORCTM_NH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(h->mu);
+  const size_t cells = (size_t)h->plane * h->nz;  // padded vector length, as orctm_nh_apply stages it
+  if (cells > kScratchCells) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "orctm_nh_apply_noalloc: %zu padded cells (pitch %d x %d x %d) exceed the staging limit of %zu",
+             cells, h->pitch, h->ny, h->nz, kScratchCells);
+    return ORCTM_NH_ERR_INVALID;
+  }
+  if (hipSetDevice(h->opts.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
+  {
+    const int crc = ensure_ctx(h);
+    if (crc != ORCTM_NH_OK) return crc;
+  }
+  double* base = nullptr;
+  hipError_t e = scratch_address(h->opts.device, &base);
+  if (e != hipSuccess) return map_err(e);
+  // k_apply64 reads x and writes y at interior cells only, and only interior cells are copied in and
+  // out, so nothing in the staging needs clearing between calls.
+  double* dx = base;
+  double* dy = base + kScratchCells;
+  hipStream_t st = h->stream;
+  const size_t es = sizeof(double);
+  e = hipMemcpy2DAsync(dx, (size_t)h->pitch * es, x, (size_t)h->nx * es, (size_t)h->nx * es, (size_t)h->ny * h->nz,
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_apply64(st, h->ctx.d_sys, h->ctx.d_tiles, h->ctx.ntiles, dx, dy);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(y, (size_t)h->nx * es, dy, (size_t)h->pitch * es, (size_t)h->nx * es, (size_t)h->ny * h->nz,
+                         hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   return map_err(e);
 }
 

@@ -199,6 +199,13 @@ class NHSolver:
         self._rc(self.lib.orctm_nh_apply(self.h, _dp(x), _dp(y)), "orctm_nh_apply")
         return y
 
+    def apply_noalloc(self, x: np.ndarray) -> np.ndarray:
+        """y = A x like apply(), staged in the library's fixed device array (orctm_nh_apply_noalloc)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._rc(self.lib.orctm_nh_apply_noalloc(self.h, _dp(x), _dp(y)), "orctm_nh_apply_noalloc")
+        return y
+
     def operator(self) -> np.ndarray:
         N = self.grid.cells
         A = np.empty((7, N))
