@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--pool", type=int, default=7, help="serve workload: grid sizes pre-assembled")
     ap.add_argument("--rate", type=float, default=200.0, help="serve workload: Poisson arrival rate (1/s)")
     ap.add_argument("--lanes", type=int, default=4, help="serve workload: server lanes")
+    ap.add_argument("--cohort-graphs", action="store_true",
+                    help="serve workload: cohorts replay captured hipGraphs instead of launching directly")
     ap.add_argument("--nsys", type=int, default=64, help="batch workload: number of systems")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
                     help="batch workload: the precisions run (both by default; one for a short profile)")
@@ -529,7 +531,7 @@ def bench_serve(args):
         return dict(n=n, rate=rate, wall=wall, lat=lat, svc=svc, classes=classes, cells=cells, cell_iters=cell_iters,
                     inner=sum(s["inner_iters"] for s in stats), rel=max(s["rel_res"] for s in stats))
 
-    with NHServer(lanes=args.lanes) as srv:
+    with NHServer(lanes=args.lanes, cohort_graphs=args.cohort_graphs) as srv:
         burst = phase(srv, args.requests, args.rate, 505)
         log(f"[bench] serve burst: {args.requests} requests in {burst['wall']:.2f} s")
         capacity = args.requests / burst["wall"]
@@ -556,7 +558,8 @@ def bench_serve(args):
         "config": {"workload": f"serving: Poisson arrivals, first {args.requests} requests at {args.rate:g}/s, then "
                                f"{args.requests2} at 60% of measured capacity; {nsteps} warm-started steps each, "
                                f"sizes log-uniform 128x128x30..2048x1536x80 on a {P}-grid pool, mixed to rel.res "
-                               f"1e-10, {args.lanes} lanes of size-class cohorts",
+                               f"1e-10, {args.lanes} lanes of size-class cohorts"
+                               + (", cohorts from captured hipGraphs" if args.cohort_graphs else ""),
                    "pool_cells": [m.grid.cells for m in models]},
         "requests": args.requests, "rounds": info["rounds"],
         "latency_ms_p50": round(pct(burst["lat"], 50), 1), "latency_ms_p99": round(pct(burst["lat"], 99), 1),

@@ -94,7 +94,11 @@ typedef struct orctm_nh_options {
   int32_t profile;      /* 1: time the inner kernels with HIP events (orctm_nh_profile_read); default 0 */
   int32_t tile_rows;    /* rows (j) per tile of the inner-iteration kernels: 0 (default) = tuned per size
                            class at create / batch create; 4, 8 or 12 = fixed. Results are the same bits
-                           for every choice (block sums are formed over fixed 4-row groups). */
+                           for every choice (block sums are formed over fixed 4-row groups). Tuned picks
+                           persist per device (arch/CU count), precision and size class in a JSON file:
+                           $ORCTM_NH_TUNE_FILE (set but empty: none), else
+                           $XDG_CACHE_HOME/orctm_nh/tile_tuning.json, else
+                           $HOME/.cache/orctm_nh/tile_tuning.json; later processes skip the timing. */
   int32_t bottom_solve; /* 1: ORCTM's CG2D4NONHY option in orctm_nh_divergence: the 2-D bottom-pressure solve
                            and the corrections of DIVG by PNH_bot before the right-hand side is formed
                            (ocvtot_nh.f90:64-147, mo_petsc.f90:147-208, 252-374, 401-538). Default 0, as
@@ -369,7 +373,12 @@ typedef struct orctm_nh_server_options {
   int32_t device;       /* HIP device ordinal (default 0) */
   int32_t lanes;        /* worker lanes (streams) running cohorts concurrently (default 4, at most 16) */
   int32_t max_cohort;   /* requests per cohort at most (default 16) */
-  int32_t reserved[6];
+  int32_t cohort_graphs; /* 1: a cohort replays its inner iterations from captured hipGraphs; 0 (default):
+                            it launches them directly. A burst of 48 mixed requests on 4 lanes served
+                            49.1 solves/s direct and 49.5 from graphs on MI355X, inside run-to-run noise;
+                            direct launches skip the capture, which holds the lock other lanes' admissions
+                            take */
+  int32_t reserved[5];
 } orctm_nh_server_options_t;
 
 typedef struct orctm_nh_request_stats {

@@ -65,7 +65,8 @@ class StatsC(ctypes.Structure):
 
 class ServerOptionsC(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_int64), ("max_active", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("lanes", ctypes.c_int32), ("max_cohort", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("lanes", ctypes.c_int32), ("max_cohort", ctypes.c_int32), ("cohort_graphs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
 
 
 class RequestStatsC(ctypes.Structure):

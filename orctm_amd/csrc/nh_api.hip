@@ -8,7 +8,10 @@
 // polled between chunks) -> FP64 refinement update, until every system has converged.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
@@ -17,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -56,6 +60,8 @@ struct Ctx {
   int device = 0;
   Sys* d_sys = nullptr;
   Tile* d_tiles = nullptr;
+  int n_otiles = 0;             // tiling of the refinement kernels (64 x kOuterRows), d_otiles
+  Tile* d_otiles = nullptr;
   Outer* d_outer = nullptr;
   void* d_inner = nullptr;
   Outer* h_outer = nullptr;  // pinned
@@ -64,6 +70,7 @@ struct Ctx {
   unsigned long long* h_seeds = nullptr;  // pinned staging of d_seeds
   hipStream_t stream = nullptr;
   bool own_stream = true;          // false: the stream belongs to the handle the context solves
+  bool use_graphs = true;          // false: chunks are launched directly (short-lived serving cohorts)
   std::vector<Group> groups;       // inner-iteration tilings, one per tile height in use
   std::vector<int> heights;        // tile height of each system
   std::vector<int> tab;            // per system: 1 = ZW/ZP read from the table and window
@@ -293,6 +300,7 @@ void ctx_destroy(Ctx& c) {
   for (auto& kv : c.graphs) hipGraphExecDestroy(kv.second);
   pool_free(c.device, c.d_sys);
   pool_free(c.device, c.d_tiles);
+  pool_free(c.device, c.d_otiles);
   for (Group& g : c.groups) {
     pool_free(c.device, g.d_tiles);
     pool_free(c.device, g.d_pairs);
@@ -342,6 +350,98 @@ hipError_t iter_fwd(Ctx& c, bool init, hipEvent_t e0 = nullptr, hipEvent_t e1 = 
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, int> g_tune;
+
+// The choices persist in a JSON file, one entry per line, keyed by the device's architecture and CU
+// count (a different device tunes afresh): ORCTM_NH_TUNE_FILE names it (empty: none), else
+// $XDG_CACHE_HOME/orctm_nh/tile_tuning.json or ~/.cache/orctm_nh/tile_tuning.json. It is read once per
+// device and process, before the first tuning, and rewritten whole (through a temporary file and a
+// rename) after each new choice. Entries of another format version are ignored: the version changes
+// whenever the kernels' relative speeds may have.
+constexpr int kTuneFormat = 2;
+std::map<int, bool> g_tune_loaded;  // device ordinal -> file read
+
+std::string tune_path() {
+  if (const char* e = getenv("ORCTM_NH_TUNE_FILE")) return std::string(e);
+  if (const char* x = getenv("XDG_CACHE_HOME"))
+    if (*x) return std::string(x) + "/orctm_nh/tile_tuning.json";
+  if (const char* h = getenv("HOME"))
+    if (*h) return std::string(h) + "/.cache/orctm_nh/tile_tuning.json";
+  return std::string();
+}
+
+std::string device_identity(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return std::string();
+  char buf[128];
+  const char* colon = strchr(prop.gcnArchName, ':');
+  const int n = colon ? (int)(colon - prop.gcnArchName) : (int)strlen(prop.gcnArchName);
+  snprintf(buf, sizeof(buf), "%.*s/%d", n, prop.gcnArchName, prop.multiProcessorCount);
+  return std::string(buf);
+}
+
+// Under g_tune_mu: entries of the file for this device into g_tune, once.
+void tune_load(int device) {
+  if (g_tune_loaded[device]) return;
+  g_tune_loaded[device] = true;
+  const std::string path = tune_path(), id = device_identity(device);
+  if (path.empty() || id.empty()) return;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[512], dev[128];
+  int ver = 0, mode = 0, cls = 0, ncls = 0, rows = 0;
+  while (fgets(line, sizeof(line), f)) {
+    if (sscanf(line, " {\"format\": %d, \"device\": \"%127[^\"]\", \"mode\": %d, \"class\": %d, \"count_class\": %d, "
+                     "\"rows\": %d}",
+               &ver, dev, &mode, &cls, &ncls, &rows) != 6)
+      continue;
+    if (ver != kTuneFormat || id != dev) continue;
+    const bool ok = mode >= 100 ? (rows == 0 || rows == 4 || rows == 8 || rows == 12) : (rows == 4 || rows == 8 || rows == 12);
+    if (ok) g_tune.emplace(std::make_tuple(device, mode, cls, ncls), rows);
+  }
+  fclose(f);
+}
+
+// Under g_tune_mu: the file rewritten with every entry this process knows for this device, plus the
+// file's entries for other devices and format versions, kept as they were.
+void tune_save(int device) {
+  const std::string path = tune_path(), id = device_identity(device);
+  if (path.empty() || id.empty()) return;
+  std::vector<std::string> keep;
+  if (FILE* f = fopen(path.c_str(), "r")) {
+    char line[512], dev[128];
+    int ver = 0;
+    while (fgets(line, sizeof(line), f))
+      if (sscanf(line, " {\"format\": %d, \"device\": \"%127[^\"]\"", &ver, dev) == 2 &&
+          (ver != kTuneFormat || id != dev)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == ',')) l.pop_back();
+        keep.push_back(l);
+      }
+    fclose(f);
+  }
+  for (const auto& kv : g_tune) {
+    if (std::get<0>(kv.first) != device) continue;
+    char buf[256];
+    snprintf(buf, sizeof(buf),
+             "{\"format\": %d, \"device\": \"%s\", \"mode\": %d, \"class\": %d, \"count_class\": %d, \"rows\": %d}",
+             kTuneFormat, id.c_str(), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), kv.second);
+    keep.push_back(buf);
+  }
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {  // create the directory chain (ignore failures: no cache then)
+    for (size_t q = 1; q <= slash; ++q)
+      if (path[q] == '/' || q == slash) mkdir(path.substr(0, q == slash ? slash : q).c_str(), 0755);
+  }
+  const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  fprintf(f, "{\"entries\": [\n");
+  for (size_t q = 0; q < keep.size(); ++q) fprintf(f, "%s%s\n", keep[q].c_str(), q + 1 < keep.size() ? "," : "");
+  fprintf(f, "]}\n");
+  const bool ok = fclose(f) == 0;
+  if (ok) rename(tmp.c_str(), path.c_str());
+  else remove(tmp.c_str());
+}
 
 int ceil_log2(long long v) {
   int c = 0;
@@ -488,6 +588,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
     int h = 0;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
+      tune_load(c.device);
       auto it = g_tune.find(key);
       if (it != g_tune.end()) h = it->second;
     }
@@ -506,6 +607,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       }
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune[key] = h;
+      tune_save(c.device);
     }
     for (int q : kv.second) c.heights[q] = h;
   }
@@ -515,6 +617,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
     int code = -1;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
+      tune_load(c.device);
       auto it = g_tune.find(key);
       if (it != g_tune.end()) code = it->second;
     }
@@ -537,6 +640,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
       }
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune[key] = code;
+      tune_save(c.device);
     }
     if (code > 0)
       for (int q : tuned) c.heights[q] = code;
@@ -586,10 +690,12 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   c.max_outer = max_outer;
   c.device = device;
   for (const Sys& s : systems) c.maxit = std::max(c.maxit, s.maxit);
-  std::vector<Tile> nat;
+  std::vector<Tile> nat, onat;
   for (int q = 0; q < c.nsys; ++q) order_into(nat, systems[q], q, TY);
-  std::vector<Tile> ord = xcd_order(nat);
+  for (int q = 0; q < c.nsys; ++q) order_into(onat, systems[q], q, kOuterRows);
+  std::vector<Tile> ord = xcd_order(nat), oord = xcd_order(onat);
   c.ntiles = (int)ord.size();
+  c.n_otiles = (int)oord.size();
   if (stream) {
     c.stream = stream;
     c.own_stream = false;
@@ -630,6 +736,9 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   HIPCHK(pool_alloc(device, sizeof(Tile) * std::max(c.ntiles, 1), &p));
   c.d_tiles = static_cast<Tile*>(p);
   if (c.ntiles) HIPCHK(put_async(c.d_tiles, ord.data(), sizeof(Tile) * c.ntiles, c.stream));
+  HIPCHK(pool_alloc(device, sizeof(Tile) * std::max(c.n_otiles, 1), &p));
+  c.d_otiles = static_cast<Tile*>(p);
+  if (c.n_otiles) HIPCHK(put_async(c.d_otiles, oord.data(), sizeof(Tile) * c.n_otiles, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return ctx_heights(c, dev, forced);
 }
@@ -642,6 +751,13 @@ hipError_t ctx_chunk(Ctx& c, int n) {
     for (int it = 0; it < n; ++it) {
       HIPCHK(iter_bwd(c, 0, c.kev[4 * it], c.kev[4 * it + 1]));
       HIPCHK(iter_fwd(c, false, c.kev[4 * it + 2], c.kev[4 * it + 3]));
+    }
+    return hipSuccess;
+  }
+  if (!c.use_graphs) {
+    for (int it = 0; it < n; ++it) {
+      HIPCHK(iter_bwd(c, 0));
+      HIPCHK(iter_fwd(c, false));
     }
     return hipSuccess;
   }
@@ -720,7 +836,7 @@ int chunk_length(Ctx& c, bool first) {
 hipError_t ctx_solve(Ctx& c, float* ms) {
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   HIPCHK(launch_begin(c.stream, c.fp32, c.d_sys, c.nsys));
-  HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
+  HIPCHK(launch_residual(c.stream, c.fp32, kOuterRows, c.d_sys, c.d_otiles, c.n_otiles, c.nsys));
   for (int m = 0; m <= c.max_outer + 1; ++m) {
     HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -770,8 +886,8 @@ hipError_t ctx_solve(Ctx& c, float* ms) {
       }
       if (!run) break;
     }
-    HIPCHK(launch_refine(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles));
-    HIPCHK(launch_residual(c.stream, c.fp32, c.d_sys, c.d_tiles, c.ntiles, c.nsys));
+    HIPCHK(launch_refine(c.stream, c.fp32, TY, c.d_sys, c.d_tiles, c.ntiles));
+    HIPCHK(launch_residual(c.stream, c.fp32, kOuterRows, c.d_sys, c.d_otiles, c.n_otiles, c.nsys));
   }
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipMemcpyAsync(c.h_outer, c.d_outer, sizeof(Outer) * c.nsys, hipMemcpyDeviceToHost, c.stream));
@@ -1772,6 +1888,10 @@ int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
 }  // extern "C"
 
 int onh::handle_precision(const orctm_nh_t* h) { return h ? (h->fp32 ? ORCTM_NH_MIXED : ORCTM_NH_FP64) : -1; }
+
+void onh::batch_use_graphs(orctm_nh_batch_t* b, bool on) {
+  if (b) b->ctx.use_graphs = on;
+}
 
 int onh::batch_perturb(orctm_nh_batch_t* b, double amplitude, const uint64_t* seeds) {
   if (!b || !seeds) return ORCTM_NH_ERR_INVALID;

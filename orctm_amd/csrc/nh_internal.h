@@ -12,6 +12,11 @@ namespace onh {
 // numbers as orctm_nh_perturb_rhs with that seed), in one launch over the batch; synchronous.
 int batch_perturb(orctm_nh_batch_t* b, double amplitude, const uint64_t* seeds);
 
+// Whether a batch replays its inner iterations from captured hipGraphs (the default) or launches them
+// directly: a serving cohort lives for a few solves, where a capture (which also holds the process-wide
+// device lock the other lanes' admissions need) costs more than the launches it saves.
+void batch_use_graphs(orctm_nh_batch_t* b, bool on);
+
 // The precision mode of a handle (ORCTM_NH_MIXED or ORCTM_NH_FP64); -1 for a null handle.
 int handle_precision(const orctm_nh_t* h);
 

@@ -34,8 +34,15 @@ hipError_t launch_guess(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, i
                         int JE);
 // Per-system outer-state initialisation (nsys threads).
 hipError_t launch_begin(hipStream_t st, bool fp32, const Sys* d_sys, int nsys);
+// Rows per tile of the residual kernel: 64 x kOuterRows blocks. It holds 112 registers per thread (4 waves
+// per SIMD): blocks of 4 wavefronts fill 16 waves per CU where 12-row blocks (12 wavefronts) leave room for
+// only one (1024x768x60: 308 -> 285 us FP32, 382 -> 365 us FP64). The update kernel k_refine (44
+// registers, 8 waves per SIMD) runs on the set-up tiling, where it is faster (194 against 203 us).
+// h = kOuterRows or TY (the set-up tiling).
+constexpr int kOuterRows = 4;
 // Each phase launch below is followed by its one-block-per-system reduction (nsys blocks).
-hipError_t launch_residual(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int nsys);
+hipError_t launch_residual(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                           int nsys);
 // Inner iteration kernels on tiles of 64 x h columns (h in kTileHeights; d_tiles built for that h).
 // Each launch over one group of tiles; the per-system reduction of the phase follows once all
 // groups of the phase have been launched. init: x = 0 and the first forward elimination.
@@ -48,7 +55,7 @@ hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_
 // raw = 1 (verification only): q = C w without the back substitution.
 hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw);
 hipError_t launch_inner_reduce(hipStream_t st, bool fp32, InnerPhase ph, const Sys* d_sys, int nsys);
-hipError_t launch_refine(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles);
+hipError_t launch_refine(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles);
 // The 7 planes of the reference operator rebuilt from the compact form (verification entry point).
 hipError_t launch_materialize(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, double* A);
 // dst <- src, bytes rounded down to 16 (bandwidth probe).

@@ -181,6 +181,7 @@ void orctm_nh_server::run_cohort(std::vector<ReqP>& cohort) {
     for (ReqP& r : members) hs.push_back(r->state);
     orctm_nh_batch_t* b = nullptr;
     int rc = orctm_nh_batch_create((int32_t)hs.size(), hs.data(), &b);
+    if (rc == ORCTM_NH_OK) onh::batch_use_graphs(b, opts.cohort_graphs != 0);
     std::vector<uint64_t> seeds(members.size());
     auto perturb_all = [&]() {
       for (size_t q = 0; q < members.size(); ++q) seeds[q] = members[q]->seed + (uint64_t)members[q]->st.steps_done;

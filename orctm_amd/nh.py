@@ -310,9 +310,10 @@ class NHServer:
     admitted by free device memory into cohorts of one size class that run concurrently on lanes."""
 
     def __init__(self, hbm_budget: int = 0, max_active: int = 64, device: int = 0, lanes: int = 0,
-                 max_cohort: int = 0):
+                 max_cohort: int = 0, cohort_graphs: bool = False):
         """lanes: worker lanes running cohorts concurrently (0 = the default, 4); max_cohort: requests
-        per cohort at most (0 = the default, 16)."""
+        per cohort at most (0 = the default, 16); cohort_graphs: replay a cohort's iterations from
+        captured hipGraphs instead of launching them directly."""
         self.lib = abi.load()
         o = abi.ServerOptionsC()
         self.lib.orctm_nh_server_default_options(ctypes.byref(o))
@@ -323,6 +324,7 @@ class NHServer:
             o.lanes = lanes
         if max_cohort:
             o.max_cohort = max_cohort
+        o.cohort_graphs = 1 if cohort_graphs else 0
         srv = ctypes.c_void_p()
         rc = self.lib.orctm_nh_server_create(ctypes.byref(o), ctypes.byref(srv))
         if rc != abi.OK:

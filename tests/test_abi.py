@@ -61,7 +61,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
               "orctm_nh_stats_t": (abi.StatsC, ["status", "tile_rows", "rel_res", "total_ms", "zw_table"]),
               "orctm_nh_profile_t": (abi.ProfileC, ["matvec_ms", "cells"]),
               "orctm_nh_server_options_t": (abi.ServerOptionsC, ["hbm_budget", "max_active", "device", "lanes",
-                                                                    "max_cohort"]),
+                                                                    "max_cohort", "cohort_graphs"]),
               "orctm_nh_request_stats_t": (abi.RequestStatsC, ["status", "rel_res", "t_done_ms"]),
               "orctm_nh_vgrid_t": (abi.VGridC, ["dlxp", "amsuo", "dtdyo", "dt", "dti"]),
               "orctm_nh_sor_grid_t": (abi.SorGridC, ["iterations", "device", "half_sweeps", "dlxp", "deute", "g",

@@ -216,15 +216,18 @@ def test_config5_largest_request_in_a_mix():
 
 
 @pytest.mark.timeout(300)
+@pytest.mark.parametrize("nx", [1024, 960])
 @pytest.mark.parametrize("zw_table", [1, -1])
 @pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
-def test_wide_shallow_grid_runs_every_streaming_form(precision, zw_table):
+def test_wide_shallow_grid_runs_every_streaming_form(precision, zw_table, nx):
     """1024x220x4 gives 16 x 55 = 880 tiles at 4 rows, 16 x 28 = 448 at 8 rows and 16 x 19 = 304 at
     12 rows: every height streams (256 tiles or more) with the paired column update, as the full-size
-    launches do, on a grid small enough for exact checks. The inner operator (against the oracle's inner
-    matvec), the line solve and the whole solve are the same bits at every height, with the ZW/ZP table
-    forced on (and the FP64 flag bits derived from the wet depths) and with the per-cell planes."""
-    g, divg = synth.build(synth.GridSpec(1024, 220, 4, dz0=20.0, dz1=60.0, depth=160.0, dmin=10.0, dmax=160.0,
+    launches do, on a grid small enough for exact checks. 960 columns make 15 tiles across (825, 420 and
+    285 tiles): the last tile of each row has no partner, and half the lanes of its paired block idle.
+    The inner operator (against the oracle's inner matvec), the line solve and the whole solve are the
+    same bits at every height, with the ZW/ZP table forced on (and the FP64 flag bits derived from the
+    wet depths) and with the per-cell planes."""
+    g, divg = synth.build(synth.GridSpec(nx, 220, 4, dz0=20.0, dz1=60.0, depth=160.0, dmin=10.0, dmax=160.0,
                                          smooth=6.0, seed=synth.BASE_SEED + 91))
     dt = np.float32 if precision == abi.MIXED else np.float64
     x = synth.normal(31, 4, g.cells).astype(dt)
