@@ -393,6 +393,10 @@ def bench_batch(args):
     prof = mx["prof"]
     # bytes moved by the matvec launches of one profiled batch solve / their summed device time
     mv_gbs = mx["mv_bytes"] / (prof["matvec_ms"] * 1e-3) / 1e9
+    # the counted traffic of the 64-system mixed batch (rocprofv3 summary in profiles/)
+    traffic = None
+    if "mixed" in out and args.nsys == 64 and args.scale == 1.0 and args.tile_rows in (0, 12) and args.zw_table == 0:
+        traffic = measured_traffic("traffic_k_bwd_f32_batch_64systems")
     line = {
         "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * mx["wall"] / args.steps, 3),
@@ -410,9 +414,11 @@ def bench_batch(args):
         "tile_rows_mixed": mx["heights"], "tile_rows_fp64": f64["heights"] if f64 else None,
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "bytes_per_launch": MATVEC_BYTES[4] * cells,
                      "note": "the bare matvec bytes (36 B per cell, FP32) per active cell-iteration over the "
-                             "summed matvec launch time"},
+                             "summed matvec launch time; bytes_per_launch and traffic (HBM bytes counted by "
+                             "FETCH_SIZE/WRITE_SIZE) are those of one launch in which every system iterates"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -133,6 +133,46 @@ def test_warm_start_resolve_is_idempotent():
     assert bits_equal(g.interior(pnh), p1)
 
 
+def test_warm_start_after_any_number_of_refinement_steps():
+    """For every case, both precisions and inner tolerances that take 1 to 5 refinement steps (both
+    parities of the step count occur): get_solution returns the solved field (oracle rel.res, direct
+    solve), a warm re-solve finds nothing left to do and returns the same bits, and a perturbed
+    warm-started step equals the same step on a fresh handle given that solution by set_guess, bit for
+    bit and with the same iteration counts."""
+    parities = set()
+    # looser inner tolerances take more refinement steps
+    configs = [(abi.MIXED, None), (abi.MIXED, 1e-2), (abi.FP64, None), (abi.FP64, 1e-3)]
+    for name in ("regional", "closed", "cyclic_x", "transect"):
+        g, divg, A, S, b = case(name)
+        for prec, rin in configs:
+            print(f"[warm steps] {name} precision={prec} rtol_inner={rin}", flush=True)
+            zero = lambda: np.zeros((g.IE, g.JE, g.KE), order="F")
+            with NHSolver(g, options(precision=prec, rtol_inner=rin)) as s:
+                s.set_rhs(divg)
+                st1 = s.solve_resident()
+                assert st1["status"] == "OK", st1
+                parities.add(st1["outer_iters"] % 2)
+                p1 = s.get_solution(zero())
+                assert oracle.rel_res(g, A, g.interior(p1), b) <= RTOL
+                pd = direct(name)
+                assert np.linalg.norm(g.interior(p1) - pd) / np.linalg.norm(pd) <= SOL_TOL
+                st2 = s.solve_resident()
+                assert st2["inner_iters"] == 0 and bits_equal(s.get_solution(zero()), p1), st2
+                s.perturb_rhs(0.01, 99)
+                st3 = s.solve_resident()
+                assert st3["status"] == "OK" and st3["inner_iters"] > 0, st3
+                p3 = s.get_solution(zero())
+            with NHSolver(g, options(precision=prec, rtol_inner=rin)) as f:
+                f.set_rhs(divg)
+                f.perturb_rhs(0.01, 99)
+                f.set_guess(p1)
+                st4 = f.solve_resident()
+                p4 = f.get_solution(zero())
+            assert st4["inner_iters"] == st3["inner_iters"] and st4["outer_iters"] == st3["outer_iters"]
+            assert bits_equal(p3, p4), (name, prec, rin)
+    assert parities == {0, 1}, parities
+
+
 def test_set_guess_and_cold_start_options():
     g, divg, A, S, b = case("cyclic_x")
     pd = direct("cyclic_x")
