@@ -507,7 +507,9 @@ struct TuneSet {
     if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
     return e;
   }
-  // one warm round, then 3 timed rounds of (matvec, column update), heights h[q] per system
+  // one warm round, then 3 timed rounds of (matvec, column update), heights h[q] per system; the
+  // fastest round counts, so that kernels of other streams on the device (serving lanes) sway the
+  // choice less
   hipError_t time(const std::vector<int>& h, float* ms) {
     std::vector<Tile*> d_tiles, d_pairs;
     std::vector<int> nts, nps, hs;
@@ -533,16 +535,21 @@ struct TuneSet {
       if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
       if (e != hipSuccess) break;
     }
+    float best = -1.f;
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
-      if (rep == 1) e = hipEventRecord(c.ev0, c.stream);
+      if (rep > 0) e = hipEventRecord(c.ev0, c.stream);
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
         e = launch_bwd(c.stream, c.fp32, hs[g], d_ts, d_tiles[g], nts[g], 0);
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
         e = launch_fwd(c.stream, c.fp32, false, hs[g], d_ts, d_tiles[g], nts[g], d_pairs[g], nps[g]);
+      if (rep == 0 || e != hipSuccess) continue;
+      float t = 0.f;
+      e = hipEventRecord(c.ev1, c.stream);
+      if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, c.ev0, c.ev1);
+      if (e == hipSuccess && (best < 0.f || t < best)) best = t;
     }
-    if (e == hipSuccess) e = hipEventRecord(c.ev1, c.stream);
-    if (e == hipSuccess) e = hipEventSynchronize(c.ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(ms, c.ev0, c.ev1);
+    *ms = best;
     hipStreamSynchronize(c.stream);
     for (Tile* d : d_tiles) pool_free(c.device, d);
     for (Tile* d : d_pairs) pool_free(c.device, d);
@@ -556,9 +563,9 @@ struct TuneSet {
 // (1) per size class, timing the class's systems alone at each height; (2) for a context of several
 // systems, the whole launch is timed under the per-class choice and under each uniform height (all
 // systems run together, so a class that prefers small tiles alone may not inside a large batch),
-// and the fastest assignment is kept. Both choices are cached for the life of the process, keyed
-// by device, precision, size class (ceil(log2) of the column count; of the total for stage 2) and
-// ceil(log2) of the number of systems.
+// and the fastest assignment is kept. Both choices are cached for the life of the process and in
+// the tuning file (tune_path), keyed by device, precision, size class (ceil(log2) of the column
+// count; of the total for stage 2) and ceil(log2) of the number of systems.
 // Tiles of 64 x 12 needed before a set of systems fills the device by itself (8 blocks per CU):
 // beyond it the tallest tile (fewest halo re-reads) is the fastest and no timing is run.
 constexpr long long kFillTiles = 8 * 256;

@@ -94,7 +94,13 @@ hipError_t pool_alloc(int device, size_t bytes, void** p) {
     *p = b;
     return hipSuccess;
   }
-  // grow: a new slab (large slabs get a spare 2 MiB so that their base can be aligned)
+  // grow: a new slab (large slabs get a spare 2 MiB so that their base can be aligned), on `device`
+  // whatever device the calling thread has current
+  int cur = -1;
+  if (!host && hipGetDevice(&cur) == hipSuccess && cur != device) {
+    const hipError_t es = hipSetDevice(device);
+    if (es != hipSuccess) return es;
+  }
   size_t want = host ? kHostSlab : small ? kSmallSlab : std::max(len, kLargeSlab);
   void* raw = nullptr;
   hipError_t e = host ? hipHostMalloc(&raw, want, hipHostMallocDefault) : hipMalloc(&raw, small ? want : want + kPage);
@@ -103,6 +109,7 @@ hipError_t pool_alloc(int device, size_t bytes, void** p) {
     want = len;
     e = hipMalloc(&raw, want + kPage);
   }
+  if (cur >= 0 && cur != device) hipSetDevice(cur);
   if (e != hipSuccess) return e;
   auto s = std::make_unique<Slab>();
   s->raw = raw;
