@@ -201,8 +201,9 @@ int orctm_nh_clone(orctm_nh_t* src, orctm_nh_t** out);
 int orctm_nh_perturb_rhs(orctm_nh_t* h, double amplitude, uint64_t seed);
 
 /* Device time of the two inner-iteration kernels, accumulated over the launches that did work
- * (HIP events recorded on the solver's own stream, inside the captured iteration graph). Needs
- * options.profile = 1. No counterpart in the reference (PETSc's -log_view is never enabled). */
+ * (HIP events recorded on the solver's own stream around each kernel; with timing on the iterations
+ * are launched directly, not replayed from the captured graphs). Needs options.profile = 1 (or
+ * orctm_nh_set_profile). No counterpart in the reference (PETSc's -log_view is never enabled). */
 typedef struct orctm_nh_profile {
   double matvec_ms;         /* fused search-direction update + 7-point matvec + p.q */
   double update_ms;         /* x/r update + vertical-line solve + r.z, r.r */
