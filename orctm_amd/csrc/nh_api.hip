@@ -28,6 +28,7 @@
 #include "nh_bot.h"
 #include "nh_kernels.h"
 #include "nh_lock.h"
+#include "nh_tiles.h"
 #include "nh_ocv.h"
 #include "nh_internal.h"
 #include "nh_pool.h"
@@ -284,15 +285,9 @@ std::vector<Tile> system_tiles(int sys_index, int nx, int ny, int h) {
   return t;
 }
 
-// Block b runs on XCD b % 8 (round-robin dispatch); give each XCD a contiguous run of tiles so
-// that vertically and horizontally adjacent tiles share an L2.
-std::vector<Tile> xcd_order(const std::vector<Tile>& nat) {
-  const int T = (int)nat.size(), X = 8;
-  std::vector<Tile> out(T);
-  std::vector<int> start(X + 1, 0);
-  for (int x = 0; x < X; ++x) start[x + 1] = start[x] + (T > x ? (T - x + X - 1) / X : 0);
-  for (int b = 0; b < T; ++b) out[b] = nat[start[b % X] + b / X];
-  return out;
+// The order of a streaming launch's tiles (nh_tiles.h): contiguous runs of equal work per XCD.
+std::vector<Tile> balanced_order(const std::vector<Tile>& nat, const std::vector<Sys>& sys) {
+  return onh::balanced_order(nat, [&](const Tile& t) { return sys[t.sys].nz; }, kDeepTiles, Tile{-1, 0, 0, 0});
 }
 
 // The caller has synchronised the context's stream (no queued work uses its blocks).
@@ -519,7 +514,7 @@ struct TuneSet {
       for (size_t q = 0; q < ts.size(); ++q)
         if (h[q] == hc) order_into(nat, ts[q], (int)q, hc);
       if (nat.empty()) continue;
-      std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
+      std::vector<Tile> ord = balanced_order(nat, ts), prs = balanced_order(pair_tiles(nat), ts);
       void *d = nullptr, *dp = nullptr;
       e = pool_alloc(c.device, sizeof(Tile) * ord.size(), &d);
       if (e != hipSuccess) break;
@@ -657,7 +652,7 @@ hipError_t ctx_heights(Ctx& c, const std::vector<Sys>& dev, const std::vector<in
     for (int q = 0; q < c.nsys; ++q)
       if (c.heights[q] == hc) order_into(nat, dev[q], q, hc);
     if (nat.empty()) continue;
-    std::vector<Tile> ord = xcd_order(nat), prs = xcd_order(pair_tiles(nat));
+    std::vector<Tile> ord = balanced_order(nat, dev), prs = balanced_order(pair_tiles(nat), dev);
     c.groups.push_back(Group{});
     Group& g = c.groups.back();
     g.h = hc;
@@ -700,7 +695,7 @@ hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<
   std::vector<Tile> nat, onat;
   for (int q = 0; q < c.nsys; ++q) order_into(nat, systems[q], q, TY);
   for (int q = 0; q < c.nsys; ++q) order_into(onat, systems[q], q, kOuterRows);
-  std::vector<Tile> ord = xcd_order(nat), oord = xcd_order(onat);
+  std::vector<Tile> ord = xcd_order(nat), oord = balanced_order(onat, systems);
   c.ntiles = (int)ord.size();
   c.n_otiles = (int)oord.size();
   if (stream) {

@@ -48,6 +48,10 @@ hipError_t launch_residual(hipStream_t st, bool fp32, int h, const Sys* d_sys, c
 // groups of the phase have been launched. init: x = 0 and the first forward elimination.
 constexpr int kTileHeights[3] = {4, 8, 12};
 enum InnerPhase { PH_INIT = 0, PH_BWD = 1, PH_FWD = 2 };
+// A launch of fewer tiles than this is a few chains of dependent levels, and the kernels take their
+// deep-prefetch or chunked forms. Tile lists of streaming launches (this many real tiles or more) may
+// hold empty tiles (sys < 0), which the streaming kernels skip; the count passed is the list's length.
+constexpr int kDeepTiles = 256;
 // d_pairs: the tiles of d_tiles with an even bx (one block runs that tile and the next one in x), for
 // launches of kDeepTiles tiles or more.
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,

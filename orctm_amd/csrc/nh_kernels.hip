@@ -703,6 +703,7 @@ __global__ void k_begin(const Sys* __restrict__ systems, int nsys) {
 template <typename T, int H>
 __global__ __launch_bounds__(64 * H) void k_residual(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   if (o->odone != RUN) return;
@@ -798,6 +799,7 @@ __global__ __launch_bounds__(64 * H) void k_residual(const Sys* __restrict__ sys
 template <typename T, int H>
 __global__ __launch_bounds__(64 * H) void k_refine(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   const Outer* o = s.outer;
   if (o->odone != RUN) return;
@@ -840,6 +842,7 @@ __global__ __launch_bounds__(64 * H) void k_refine(const Sys* __restrict__ syste
 template <typename T, bool INIT, int H, int U>
 __global__ __launch_bounds__(64 * H) void k_fwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
@@ -997,6 +1000,7 @@ template <typename T, bool INIT, int H, int U>
 __global__ __launch_bounds__(64 * H) void k_fwd2(const Sys* __restrict__ systems, const Tile* __restrict__ tiles) {
   using T2 = typename V2<T>::type;
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
@@ -1118,7 +1122,6 @@ constexpr int bwd_threads() {
 // (kDeepTiles) is a few chains of dependent levels: FP32 prefetches 4 levels there, FP64 runs the
 // chunked kernel below, and k_fwd loads longer batches.
 constexpr int kDeepDepth = 4;
-constexpr int kDeepTiles = 256;
 
 // With the table form, whether the matvec kernels derive each cell's flag bits from the column's wet
 // depths (2 B per cell less) or read the flag plane: FP64 derives (regional FP64 20.6 -> 20.9 solves/s,
@@ -1134,6 +1137,7 @@ template <typename T, int H, int D>
 __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
                                                 int raw) {
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   Outer* o = s.outer;
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
@@ -1326,6 +1330,7 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd_ch(const Sys* __restri
   static_assert(((CH + 1) & CH) == 0, "CH + 1 must be a power of two");
   constexpr int NS = CH + 1;
   const Tile t = tiles[blockIdx.x];
+  if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
   if (in->done != RUN) return;
