@@ -180,8 +180,9 @@ def main():
                     help="troneu workload: half-sweeps per launch, 0 = tuned, or 4, 6, 7, 8, 12")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (systems would shard "
-              "across GPUs with no exchange; see the design notes).", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (independent systems "
+              "would shard whole across GPUs with no exchange, one large system by x-slabs with a one-cell "
+              "halo face per matvec; neither is built).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":
