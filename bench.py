@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, help="inner-kernel tile height: 0 = tuned, or 4, 8, 12")
     ap.add_argument("--zw-table", type=int, default=0, choices=[-1, 0, 1],
                     help="where the iteration kernels read ZW/ZP: 0 = by size, 1 = table, -1 = per-cell planes")
+    ap.add_argument("--buffer-addressing", type=int, default=0, choices=[-1, 0],
+                    help="matvec memory addressing: 0 = buffer descriptors where the arrays fit, -1 = 64-bit pointers")
     ap.add_argument("--sor-depth", type=int, default=0,
                     help="troneu workload: half-sweeps per launch, 0 = tuned, or 4, 6, 7, 8, 12")
     args = ap.parse_args()
@@ -207,7 +209,8 @@ def main():
     sols = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         # timed solves: the production path (inner iterations replayed from the captured hipGraph)
-        s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table))
+        s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table,
+                                buffer_addressing=args.buffer_addressing))
         s.set_rhs(divg)
         log(f"[bench] {name}: device bytes {s.device_bytes() / 1e9:.2f} GB")
         wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
@@ -277,11 +280,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (7-point matvec + back substitution + p update + p.q)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("traffic_k_bwd_f32_regional_1024x768x60"),
+                     "traffic": (measured_traffic("traffic_k_bwd_f32_regional_1024x768x60")
+                                 if args.buffer_addressing == 0 else None),
                      "bytes_per_launch": mv_bytes, "bytes_per_cell": MATVEC_BYTES[4],
                      "bytes_note": "the bare matvec y = A x: 7 coefficients + x + y per cell (FP32)",
                      "kernel_bytes_per_launch": kern_bytes, "kernel_GBps": round(kern_bytes / (mv_avg_ms * 1e-3) / 1e9, 1),
-                     "zw_table": int(table), "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
+                     "zw_table": int(table), "buffer_addressing": mx["stats"][-1].get("buffer_addressing", 0),
+                     "avg_launch_ms": round(mv_avg_ms, 5), "launches": prof["matvec_launches"]},
         "other_kernels": {
             "k_fwd<float>": {"avg_launch_ms": round(up_avg_ms, 5), "bytes_per_cell": 20, "bytes_per_column": 4,
                              "GBps": round(fwd_bytes(cells, columns, 4) / (up_avg_ms * 1e-3) / 1e9, 1)},
@@ -337,7 +342,8 @@ def bench_batch(args):
     for name, prec in precs:
         solvers = []
         for g, divg in systems:
-            s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table))
+            s = NHSolver(g, options(precision=prec, warm_start=False, tile_rows=args.tile_rows, zw_table=args.zw_table,
+                                    buffer_addressing=args.buffer_addressing))
             s.set_rhs(divg)
             solvers.append(s)
         batch = NHBatch(solvers)
@@ -396,7 +402,8 @@ def bench_batch(args):
     mv_gbs = mx["mv_bytes"] / (prof["matvec_ms"] * 1e-3) / 1e9
     # the counted traffic of the 64-system mixed batch (rocprofv3 summary in profiles/)
     traffic = None
-    if "mixed" in out and args.nsys == 64 and args.scale == 1.0 and args.tile_rows in (0, 12) and args.zw_table == 0:
+    if "mixed" in out and args.nsys == 64 and args.scale == 1.0 and args.tile_rows in (0, 12) and args.zw_table == 0 \
+            and args.buffer_addressing == 0:
         traffic = measured_traffic("traffic_k_bwd_f32_batch_64systems")
     line = {
         "metric": METRIC, "value": round(n * args.steps / mx["wall"], 3), "unit": "solves/s", "n_gpus": 1,
@@ -444,7 +451,8 @@ def bench_transect(args):
     cpu_s = (time.perf_counter() - t) / reps
     res = {}
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
-        s = NHSolver(g, options(precision=prec, warm_start=False, zw_table=args.zw_table))
+        s = NHSolver(g, options(precision=prec, warm_start=False, zw_table=args.zw_table,
+                                buffer_addressing=args.buffer_addressing))
         s.set_rhs(divg)
         wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
@@ -494,7 +502,8 @@ def bench_serve(args):
         if args.scale != 1.0:
             nx, ny = max(8, int(nx * args.scale)), max(8, int(ny * args.scale))
         g, divg = synth.build(synth.GridSpec(nx, ny, nz, seed=synth.BASE_SEED + 500 + q))
-        m = NHSolver(g, options(precision=abi.MIXED, warm_start=True, zw_table=args.zw_table))
+        m = NHSolver(g, options(precision=abi.MIXED, warm_start=True, zw_table=args.zw_table,
+                                buffer_addressing=args.buffer_addressing))
         m.set_rhs(divg)
         for a in ("dlxp", "dlyp", "dlxu", "dlyv", "ddpo", "weto"):
             setattr(g, a, None)

@@ -109,7 +109,13 @@ typedef struct orctm_nh_options {
                            least 2^20 cells (one system, or a batch in total), else the per-cell planes;
                            1 = the table whenever it reproduces every column; -1 = always the planes.
                            Results are the same bits for every choice. */
-  int32_t reserved[2];
+  int32_t buffer_addressing; /* how the matvec kernel addresses memory (stats.buffer_addressing reports the
+                           choice): 0 (default) = through buffer descriptors, with masked lanes and
+                           wavefronts turned away by the hardware's range check instead of branches,
+                           when every inner array of the solve launches holds at most 2^31 bytes (a
+                           system of at most 2^29 padded cells in FP32, 2^28 in FP64), else through
+                           64-bit pointers; -1 = always 64-bit pointers. Same bits either way. */
+  int32_t reserved[1];
 } orctm_nh_options_t;
 
 /* Per-solve statistics. */
@@ -125,7 +131,7 @@ typedef struct orctm_nh_stats {
   int32_t zw_table;     /* 1: the iteration kernels read ZW/ZP from the wet column's per-level table and
                            each column's 3-level window around its bottom (checked bit for bit at create);
                            0: from the per-cell planes */
-  int32_t reserved_i;
+  int32_t buffer_addressing; /* 1: the matvec kernel addressed memory through buffer descriptors */
   double reserved[3];
 } orctm_nh_stats_t;
 

@@ -36,7 +36,7 @@ class OptionsC(ctypes.Structure):
         ("warm_start", ctypes.c_int32), ("rtol", ctypes.c_double), ("rtol_inner", ctypes.c_double),
         ("check_every", ctypes.c_int32), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("tile_rows", ctypes.c_int32), ("bottom_solve", ctypes.c_int32), ("zw_table", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("buffer_addressing", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -53,14 +53,15 @@ class StatsC(ctypes.Structure):
         ("status", ctypes.c_int32), ("outer_iters", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
         ("tile_rows", ctypes.c_int32), ("rel_res", ctypes.c_double), ("bnorm", ctypes.c_double),
         ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("zw_table", ctypes.c_int32),
-        ("reserved_i", ctypes.c_int32), ("reserved", ctypes.c_double * 3),
+        ("buffer_addressing", ctypes.c_int32), ("reserved", ctypes.c_double * 3),
     ]
 
     def as_dict(self):
         return {"status": STATUS_NAMES[self.status] if 0 <= self.status < len(STATUS_NAMES) else self.status,
                 "outer_iters": self.outer_iters, "inner_iters": self.inner_iters, "rel_res": self.rel_res,
                 "bnorm": self.bnorm, "solve_ms": self.solve_ms, "total_ms": self.total_ms,
-                "tile_rows": self.tile_rows, "zw_table": self.zw_table}
+                "tile_rows": self.tile_rows, "zw_table": self.zw_table,
+                "buffer_addressing": self.buffer_addressing}
 
 
 class ServerOptionsC(ctypes.Structure):

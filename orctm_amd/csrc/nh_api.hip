@@ -75,6 +75,7 @@ struct Ctx {
   std::vector<Group> groups;       // inner-iteration tilings, one per tile height in use
   std::vector<int> heights;        // tile height of each system
   std::vector<int> tab;            // per system: 1 = ZW/ZP read from the table and window
+  bool buf = false;                // the matvec kernel addresses memory through buffer descriptors
   std::map<int, hipGraphExec_t> graphs;  // captured runs of 1, 2, 4, ... inner iterations
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional kernel timing: events around every matvec / update launch (direct launches)
@@ -238,8 +239,8 @@ hipError_t ensure_fort(orctm_nh_t* h) {
 }
 
 hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<int>& forced,
-                     const std::vector<int>& zw_table, bool fp32, int check_every, int max_outer, int device,
-                     bool prof, hipStream_t stream);
+                     const std::vector<int>& zw_table, const std::vector<int>& addressing, bool fp32, int check_every,
+                     int max_outer, int device, bool prof, hipStream_t stream);
 void ctx_destroy(Ctx& c);
 
 // The single-system context of a handle, built on its stream on first use: a clone that is only ever
@@ -247,8 +248,9 @@ void ctx_destroy(Ctx& c);
 int ensure_ctx(orctm_nh_t* h) {
   if (h->ctx_ready) return ORCTM_NH_OK;
   DEVLOCK;
-  hipError_t e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, {h->opts.zw_table}, h->fp32, h->opts.check_every,
-                           h->opts.max_outer, h->opts.device, h->opts.profile != 0, h->stream);
+  hipError_t e = ctx_build(h->ctx, {h->sys}, {h->opts.tile_rows}, {h->opts.zw_table}, {h->opts.buffer_addressing},
+                           h->fp32, h->opts.check_every, h->opts.max_outer, h->opts.device, h->opts.profile != 0,
+                           h->stream);
   if (e != hipSuccess) {
     hipStreamSynchronize(h->stream);
     ctx_destroy(h->ctx);
@@ -323,7 +325,7 @@ hipError_t put_async(void* dst, const void* src, size_t bytes, hipStream_t st) {
 // given, events bracket the phase's kernels (not the reduction) for kernel timing.
 hipError_t iter_bwd(Ctx& c, int raw, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (e0) HIPCHK(hipEventRecord(e0, c.stream));
-  for (const Group& g : c.groups) HIPCHK(launch_bwd(c.stream, c.fp32, g.h, c.d_sys, g.d_tiles, g.ntiles, raw));
+  for (const Group& g : c.groups) HIPCHK(launch_bwd(c.stream, c.fp32, c.buf, g.h, c.d_sys, g.d_tiles, g.ntiles, raw));
   if (e1) HIPCHK(hipEventRecord(e1, c.stream));
   return launch_inner_reduce(c.stream, c.fp32, PH_BWD, c.d_sys, c.nsys);
 }
@@ -352,7 +354,7 @@ std::map<std::tuple<int, int, int, int>, int> g_tune;
 // device and process, before the first tuning, and rewritten whole (through a temporary file and a
 // rename) after each new choice. Entries of another format version are ignored: the version changes
 // whenever the kernels' relative speeds may have.
-constexpr int kTuneFormat = 2;
+constexpr int kTuneFormat = 3;
 std::map<int, bool> g_tune_loaded;  // device ordinal -> file read
 
 std::string tune_path() {
@@ -534,7 +536,7 @@ struct TuneSet {
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
       if (rep > 0) e = hipEventRecord(c.ev0, c.stream);
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
-        e = launch_bwd(c.stream, c.fp32, hs[g], d_ts, d_tiles[g], nts[g], 0);
+        e = launch_bwd(c.stream, c.fp32, c.buf, hs[g], d_ts, d_tiles[g], nts[g], 0);
       for (size_t g = 0; g < hs.size() && e == hipSuccess; ++g)
         e = launch_fwd(c.stream, c.fp32, false, hs[g], d_ts, d_tiles[g], nts[g], d_pairs[g], nps[g]);
       if (rep == 0 || e != hipSuccess) continue;
@@ -680,9 +682,11 @@ constexpr long long kTabMinCells = 1LL << 20;
 // stream: the owning handle's stream for a single-system context, or nullptr for a context of its own
 // (a batch), which then creates one. Everything the build does is ordered on that stream.
 hipError_t ctx_build(Ctx& c, const std::vector<Sys>& systems, const std::vector<int>& forced,
-                     const std::vector<int>& zw_table, bool fp32, int check_every, int max_outer, int device,
-                     bool prof, hipStream_t stream) {
+                     const std::vector<int>& zw_table, const std::vector<int>& addressing, bool fp32, int check_every,
+                     int max_outer, int device, bool prof, hipStream_t stream) {
   c.nsys = (int)systems.size();
+  c.buf = true;
+  for (int q = 0; q < c.nsys; ++q) c.buf = c.buf && addressing[q] != -1 && bwd_buffers_fit(systems[q], fp32);
   c.prof = prof;
   c.iter_seen.assign(c.nsys, 0);
   c.lrate.assign(c.nsys, 0.0);
@@ -917,12 +921,14 @@ int outer_status(const Outer& o) {
   }
 }
 
-void fill_stats(const Outer& o, int tile_rows, int zw_table, orctm_nh_stats_t* st, double solve_ms, double total_ms) {
+void fill_stats(const Outer& o, int tile_rows, int zw_table, bool buf, orctm_nh_stats_t* st, double solve_ms,
+                double total_ms) {
   if (!st) return;
   memset(st, 0, sizeof(*st));
   st->status = outer_status(o);
   st->tile_rows = tile_rows;
   st->zw_table = zw_table;
+  st->buffer_addressing = buf ? 1 : 0;
   st->outer_iters = o.outer;
   st->inner_iters = o.inner_total;
   st->rel_res = o.rel;
@@ -1266,6 +1272,7 @@ int orctm_nh_create(const orctm_nh_grid_t* grid, const orctm_nh_options_t* opts,
   if (o.tile_rows != 0 && o.tile_rows != 4 && o.tile_rows != 8 && o.tile_rows != 12) return ORCTM_NH_ERR_INVALID;
   if (o.bottom_solve != 0 && o.bottom_solve != 1) return ORCTM_NH_ERR_INVALID;
   if (o.zw_table < -1 || o.zw_table > 1) return ORCTM_NH_ERR_INVALID;
+  if (o.buffer_addressing != 0 && o.buffer_addressing != -1) return ORCTM_NH_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device || o.device < 0) return ORCTM_NH_ERR_NODEVICE;
   if (hipSetDevice(o.device) != hipSuccess) return ORCTM_NH_ERR_NODEVICE;
@@ -1357,7 +1364,7 @@ int orctm_nh_solve_resident(orctm_nh_t* h, orctm_nh_stats_t* st) {
   float ms = 0.f;
   hipError_t e = ctx_solve(h->ctx, &ms);
   if (e != hipSuccess) return map_err(e);
-  fill_stats(h->ctx.h_outer[0], h->ctx.heights[0], h->ctx.tab[0], st, ms, now_ms() - t0);
+  fill_stats(h->ctx.h_outer[0], h->ctx.heights[0], h->ctx.tab[0], h->ctx.buf, st, ms, now_ms() - t0);
   return outer_status(h->ctx.h_outer[0]);
 }
 
@@ -1834,7 +1841,7 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
   int max_outer = 0, check = 16, dev = 0;
   bool prof = false;
   std::vector<Sys> ss;
-  std::vector<int> forced, zwt;
+  std::vector<int> forced, zwt, adr;
   for (int q = 0; q < n; ++q) {
     orctm_nh_t* h = systems[q];
     if (!h || h->fp32 != fp32 || h->opts.device != systems[0]->opts.device) return ORCTM_NH_ERR_INVALID;
@@ -1843,6 +1850,7 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
     ss.push_back(h->sys);
     forced.push_back(h->opts.tile_rows);
     zwt.push_back(h->opts.zw_table);
+    adr.push_back(h->opts.buffer_addressing);
     max_outer = std::max(max_outer, h->opts.max_outer);
     check = systems[0]->opts.check_every;
     dev = h->opts.device;
@@ -1852,7 +1860,7 @@ int orctm_nh_batch_create(int32_t n, orctm_nh_t* const* systems, orctm_nh_batch_
   if (!b) return ORCTM_NH_ERR_NOMEM;
   b->members.assign(systems, systems + n);
   hipSetDevice(dev);
-  hipError_t e = ctx_build(b->ctx, ss, forced, zwt, fp32, check, max_outer, dev, prof, nullptr);
+  hipError_t e = ctx_build(b->ctx, ss, forced, zwt, adr, fp32, check, max_outer, dev, prof, nullptr);
   if (e != hipSuccess) {
     if (b->ctx.stream) hipStreamSynchronize(b->ctx.stream);
     ctx_destroy(b->ctx);
@@ -1882,7 +1890,7 @@ int orctm_nh_batch_solve(orctm_nh_batch_t* b, orctm_nh_stats_t* st) {
   for (int q = 0; q < b->ctx.nsys; ++q) {
     const int s = outer_status(b->ctx.h_outer[q]);
     if (rc == ORCTM_NH_OK && s != ORCTM_NH_OK) rc = s;
-    if (st) fill_stats(b->ctx.h_outer[q], b->ctx.heights[q], b->ctx.tab[q], st + q, ms, tot);
+    if (st) fill_stats(b->ctx.h_outer[q], b->ctx.heights[q], b->ctx.tab[q], b->ctx.buf, st + q, ms, tot);
   }
   return rc;
 }

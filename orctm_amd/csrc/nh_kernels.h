@@ -56,8 +56,17 @@ constexpr int kDeepTiles = 256;
 // launches of kDeepTiles tiles or more.
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
                       const Tile* d_pairs, int npairs);
-// raw = 1 (verification only): q = C w without the back substitution.
-hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw);
+// raw = 1 (verification only): q = C w without the back substitution. buf: the kernel may address
+// memory through buffer descriptors, which requires every inner array of every system of the launch
+// to hold at most kBufMaxBytes (bwd_buffers_fit); else 64-bit pointers. Same bits either way.
+hipError_t launch_bwd(hipStream_t st, bool fp32, bool buf, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                      int raw);
+// The largest array the buffer-addressed matvec handles (lane offsets stay below 2^31 bytes).
+constexpr unsigned long long kBufMaxBytes = 1ull << 31;
+// Whether a system's inner arrays (nz planes in the inner precision) fit the buffer-addressed matvec.
+inline bool bwd_buffers_fit(const Sys& s, bool fp32) {
+  return (unsigned long long)s.plane * (unsigned long long)s.nz * (fp32 ? 4ull : 8ull) <= kBufMaxBytes;
+}
 hipError_t launch_inner_reduce(hipStream_t st, bool fp32, InnerPhase ph, const Sys* d_sys, int nsys);
 hipError_t launch_refine(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles);
 // The 7 planes of the reference operator rebuilt from the compact form (verification entry point).

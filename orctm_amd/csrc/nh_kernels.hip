@@ -70,16 +70,56 @@ __device__ __forceinline__ void st_nt(gptr<T> p, size_t i, T v) {
   __builtin_nontemporal_store(v, p + i);
 }
 
+// Buffer-descriptor access (the matvec kernel's column march). A descriptor covers one array; a lane
+// addresses it by a 32-bit byte offset (its column's offset plus the level's). A lane that must not
+// touch memory takes an offset at or past kOob, beyond every array of at most kBufMaxBytes: the
+// hardware's range check then returns 0 for its load and drops its store. A whole wavefront is masked
+// by a descriptor of size 0. So every wavefront issues the same memory instructions on every path, and
+// the compiler's wait counts keep the loads of the next level in flight instead of draining them where
+// branches around loads and stores join.
+constexpr uint32_t kOob = 0x80000000u;
+static_assert(kBufMaxBytes <= kOob, "offsets of valid lanes stay below kOob");
+constexpr int kBufWord3 = 0x00020000;  // gfx9 descriptor word 3: 32-bit data format, untyped access
+using Rsrc = __amdgpu_buffer_rsrc_t;
+enum : int { kAuxNone = 0, kAuxNt = 2 };  // cache policy bits: none, non-temporal
+
+__device__ __forceinline__ Rsrc buf_of(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufWord3);
+}
+template <typename E, int AUX>
+__device__ __forceinline__ E bld(Rsrc r, uint32_t vo) {
+  if constexpr (sizeof(E) == 2) {
+    return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b16(r, vo, 0, AUX));
+  } else if constexpr (sizeof(E) == 4) {
+    return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, vo, 0, AUX));
+  } else {
+    static_assert(sizeof(E) == 8, "2-, 4- or 8-byte elements");
+    return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, vo, 0, AUX));
+  }
+}
+template <typename E>
+__device__ __forceinline__ void bst_nt(Rsrc r, uint32_t vo, E v) {
+  if constexpr (sizeof(E) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, 0, kAuxNt);
+  } else {
+    static_assert(sizeof(E) == 8, "4- or 8-byte elements");
+    typedef unsigned int U2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, vo, 0, kAuxNt);
+  }
+}
+
 // Factor storage per inner precision: fp16 bits for FP32, the value itself for FP64.
 template <typename T>
 struct Fac {
   using S = T;
+  __device__ static T dec(T v) { return v; }
   __device__ static T ld(gptr<const S> p, size_t i) { return p[i]; }
   __device__ static T ld_nt(gptr<const S> p, size_t i) { return onh::ld_nt(p, i); }
 };
 template <>
 struct Fac<float> {
   using S = uint16_t;
+  __device__ static float dec(uint16_t v) { return half_val(v); }
   __device__ static float ld(gptr<const uint16_t> p, size_t i) { return half_val(p[i]); }
   __device__ static float ld_nt(gptr<const uint16_t> p, size_t i) { return half_val(onh::ld_nt(p, i)); }
 };
@@ -241,6 +281,15 @@ struct ZCol {
   __device__ T zp(int m) const {
     const int d = m - kx;
     return d == 0 ? p0 : d == 1 ? p1 : d == 2 ? p2 : tp[m];
+  }
+  // the window's value at level m where the window covers m, else t (the table's entry)
+  __device__ T zw_or(int m, T t) const {
+    const int d = m - kx;
+    return d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : t;
+  }
+  __device__ T zp_or(int m, T t) const {
+    const int d = m - kx;
+    return d == 0 ? p0 : d == 1 ? p1 : d == 2 ? p2 : t;
   }
 };
 
@@ -1132,36 +1181,40 @@ template <typename T>
 constexpr bool kDeriveFlags = sizeof(T) == 8;
 
 
+// Occupancy the buffer form of the FP32 streaming matvec asks the compiler for: two 12-row blocks
+// (15 wavefronts each) per CU need 8 wavefronts per SIMD, four 4-row blocks (7 each) 7; it reaches them
+// by spilling a few scalar registers into vector-register lanes. The other forms keep the compiler's
+// own choice.
+template <typename T, int H, int D, bool BUF>
+constexpr int bwd_waves() {
+  return (BUF && sizeof(T) == 4 && D == 1) ? (H == 12 ? 8 : H == 4 ? 7 : 1) : 1;
+}
+
 // raw != 0 (verification only): w is taken as z itself (l treated as 0), so q = C w.
-template <typename T, int H, int D>
-__global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles,
-                                                int raw) {
+// BUF: the column march addresses memory through buffer descriptors (every array of the launch at most
+// kBufMaxBytes); else through 64-bit pointers with branches around the masked accesses. Same bits.
+template <typename T, int H, int D, bool BUF>
+__global__ __launch_bounds__(bwd_threads<H>()) __attribute__((amdgpu_waves_per_eu(bwd_waves<T, H, D, BUF>())))
+void k_bwd(const Sys* __restrict__ systems, const Tile* __restrict__ tiles, int raw) {
   const Tile t = tiles[blockIdx.x];
   if (t.sys < 0) return;  // an empty tile of a work-balanced list
   const Sys& s = systems[t.sys];
-  Outer* o = s.outer;
   Inner<T>* in = reinterpret_cast<Inner<T>*>(s.inner);
   if (in->done != RUN) return;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // the wavefront's row and the inner state's flags as wave-uniform values (the inner state is read
+  // with vector loads), so that the masks and descriptors derived from them stay scalar
+  const int tx = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i0 = t.bx * TX, j0 = t.by * H;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const size_t P = s.plane, IP = s.pitch, NP = P * (size_t)nz;
-  const bool first = in->first != 0;
-  const int pb = in->pbuf;
+  const size_t P = s.plane, IP = s.pitch;
+  const bool first = __builtin_amdgcn_readfirstlane(in->first) != 0;
+  const int pb = __builtin_amdgcn_readfirstlane(in->pbuf);
   const T beta = in->beta;
   // the previous iteration's solution update x += alpha p_old, deferred to here where p_old is read
   // anyway (not on an inner solve's first iteration, not in the verification mode)
   const bool xupd = !first && !raw;
   const T alpha = in->alpha;
-  gptr<const T> w = G<const T>(s.z);
-  gptr<const typename Fac<T>::S> L = G<const typename Fac<T>::S>(s.L);
-  gptr<T> xv = G<T>(s.x);
-  gptr<const T> pold = G<const T>(pb ? s.pb : s.pa);
-  gptr<T> pnew = G<T>(pb ? s.pa : s.pb);
-  gptr<T> q = G<T>(s.q);
-  gptr<const T> zwi = G<const T>(s.zw_in);
-  gptr<const T> zpi = G<const T>(s.zp_in);
-  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+  using FS = typename Fac<T>::S;
 
   __shared__ T lds[3][H + 2][TX + 2];
 
@@ -1201,14 +1254,18 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
   // finished with set `cur` (p at level k-1 from its w/l/p_old, q at level k from its row data), set
   // `nxt` receives w/l/p_old of level k-1-D and the row data (ZW, ZP, flags) of level k-D, the set
   // that step k-D will finish with. The 7 inner coefficients are rebuilt from the row data and this
-  // column's metric products.
+  // column's metric products. A set holds the loaded values as loaded (fp16 factor bits, the plane's
+  // or the table's ZW/ZP): they are decoded and selected where they are used, so that no load is
+  // waited for before its level.
   struct Stage {
-    T w, l, p, x;
-    T zw, zp;
+    T w, p, x;
+    FS l;
+    T zw, zp, tw, tp;
     unsigned f;
   };
   T cc[7];
   const bool tab = s.tab != 0;
+  const bool derive = tab && kDeriveFlags<T>;
   ZCol<T> zcol;
   if (centre) {
     gptr<const T> cpi = G<const T>(s.col_in);
@@ -1219,65 +1276,129 @@ __global__ __launch_bounds__(bwd_threads<H>()) void k_bwd(const Sys* __restrict_
                 off);
   }
   T zw_below = T(0);  // ZW at the bottom face of the current level (top face of the level below)
-  (void)NP;
   T zc = T(0), lc = T(0);  // z and l of this thread's column at the level below
+
+  // Memory of the march. Loads: w, l, p_old of every slot lane's column (zero-filled outside the
+  // domain), x of the centre lanes in the domain when the update is due, the row data of the centre
+  // lanes (planes without the table; with it the table's entry, the same for the whole block, by a
+  // scalar load). Stores: p, x and q of the centre lanes in the domain. Level m < 0 (past the surface,
+  // the last step's level) stores nothing.
+  const void* pold_base = pb ? s.pb : s.pa;
+  void* pnew_base = pb ? s.pa : s.pb;
+  // buffer form: descriptors and lane offsets
+  const uint32_t PBu = (uint32_t)(P * sizeof(T)), PFu = (uint32_t)(P * sizeof(FS)), P2u = (uint32_t)(P * 2);
+  const uint32_t o_col = val ? (uint32_t)(off * sizeof(T)) : kOob;
+  const uint32_t o_fac = val ? (uint32_t)(off * sizeof(FS)) : kOob;
+  const uint32_t o_h = val ? (uint32_t)(off * 2) : kOob;
+  const uint32_t o_st = active ? (uint32_t)(off * sizeof(T)) : kOob;
+  const uint32_t AB = PBu * (uint32_t)nz, AF = PFu * (uint32_t)nz, A2 = P2u * (uint32_t)nz;
+  const uint32_t rec_c = centre ? AB : 0u, rec_x = (centre && xupd) ? AB : 0u;
+  const Rsrc r_w = buf_of(s.z, AB), r_l = buf_of(s.L, AF), r_po = buf_of(pold_base, AB);
+  const Rsrc r_x = buf_of(s.x, rec_x), r_pn = buf_of(pnew_base, rec_c), r_q = buf_of(s.q, rec_c);
+  const Rsrc r_zw = buf_of(s.zw_in, tab ? 0u : rec_c), r_zp = buf_of(s.zp_in, tab ? 0u : rec_c);
+  const Rsrc r_f = buf_of(s.flg, (centre && !derive) ? A2 : 0u);
+  typedef __attribute__((address_space(4))) const T* cptr;  // read-only during the kernel: scalar loads
+  const cptr ctw = (cptr)s.tw_in, ctp = (cptr)s.tp_in;
+  // pointer form
+  gptr<const T> w = G<const T>(s.z);
+  gptr<const FS> L = G<const FS>(s.L);
+  gptr<T> xv = G<T>(s.x);
+  gptr<const T> pold = G<const T>(pold_base);
+  gptr<T> pnew = G<T>(pnew_base);
+  gptr<T> q = G<T>(s.q);
+  gptr<const T> zwi = G<const T>(s.zw_in);
+  gptr<const T> zpi = G<const T>(s.zp_in);
+  gptr<const uint16_t> flg = G<const uint16_t>(s.flg);
+
   auto load_a = [&](int m, Stage& st) {
-    const size_t om = (size_t)(m > 0 ? m : 0) * P + off;
-    st.w = w[om];
-    st.l = Fac<T>::ld(L, om);
-    st.p = pold[om];
-    if (centre && xupd) st.x = ld_nt<T>(xv, om);
+    const uint32_t mm = (uint32_t)(m > 0 ? m : 0);
+    if constexpr (BUF) {
+      st.w = bld<T, kAuxNone>(r_w, o_col + mm * PBu);
+      st.l = bld<FS, kAuxNone>(r_l, o_fac + mm * PFu);
+      st.p = bld<T, kAuxNone>(r_po, o_col + mm * PBu);
+      st.x = bld<T, kAuxNt>(r_x, o_st + mm * PBu);
+    } else {
+      const size_t om = (size_t)mm * P + off;
+      st.w = w[om];
+      st.l = L[om];
+      st.p = pold[om];
+      if (centre && xupd) st.x = ld_nt<T>(xv, om);
+    }
   };
   auto load_c = [&](int m, Stage& st) {
-    if (!centre) return;
-    const int mm = m > 0 ? m : 0;
-    const size_t cm = (size_t)mm * P + off;
-    if (tab) {
-      st.zw = zcol.zw(mm);
-      st.zp = zcol.zp(mm);
+    const uint32_t mm = (uint32_t)(m > 0 ? m : 0);
+    if constexpr (BUF) {
+      st.zw = bld<T, kAuxNt>(r_zw, o_col + mm * PBu);
+      st.zp = bld<T, kAuxNt>(r_zp, o_col + mm * PBu);
+      st.f = bld<uint16_t, kAuxNt>(r_f, o_h + mm * P2u);
+      st.tw = tab ? ctw[mm] : T(0);
+      st.tp = tab ? ctp[mm] : T(0);
     } else {
-      st.zw = ld_nt<T>(zwi, cm);
-      st.zp = ld_nt<T>(zpi, cm);
+      if (!centre) return;
+      const size_t cm = (size_t)mm * P + off;
+      if (tab) {
+        st.zw = zcol.zw(mm);
+        st.zp = zcol.zp(mm);
+      } else {
+        st.zw = ld_nt<T>(zwi, cm);
+        st.zp = ld_nt<T>(zpi, cm);
+      }
+      st.f = derive ? zcol.flag(mm, nz) : ld_nt<uint16_t>(flg, cm);
     }
-    st.f = (tab && kDeriveFlags<T>) ? zcol.flag(mm, nz) : ld_nt<uint16_t>(flg, cm);
+  };
+  auto store = [&](Rsrc r, gptr<T> a, int m, T v) {
+    if constexpr (BUF) {
+      bst_nt<T>(r, m >= 0 ? o_st + (uint32_t)m * PBu : kOob, v);
+    } else {
+      if (active && m >= 0) st_nt<T>(a, (size_t)m * P + off, v);
+    }
   };
   // back substitution and search-direction update of level m for this thread's column
   auto level = [&](int m, const Stage& st) -> T {
     const T zm = fmad(-lc, zc, st.w);
     const T pm = first ? zm : fmad(beta, st.p, zm);
     zc = zm;
-    lc = raw ? T(0) : st.l;
+    lc = raw ? T(0) : Fac<T>::dec(st.l);
     const T pout = val ? pm : T(0);
-    if (slot) lds[m % 3][sy][sx] = pout;
-    if (active) {
-      st_nt<T>(pnew, (size_t)m * P + off, pm);
-      if (xupd) st_nt<T>(xv, (size_t)m * P + off, fmad(alpha, st.p, st.x));
-    }
+    if (slot) lds[(m + 3) % 3][sy][sx] = pout;
+    store(r_pn, pnew, m, pm);
+    if (BUF || xupd) store(r_x, xv, m, fmad(alpha, st.p, st.x));
     return pout;
   };
   T p0 = T(0), pp1 = T(0), dot = T(0);
+  // step k: row k with p of levels k+1, k, k-1; level k-1 is advanced first. On the last step (k = 0)
+  // the buffer form runs the level past the surface with its stores masked and takes its p as 0, so
+  // that the step's memory instructions are those of every other step; the pointer form skips it.
   auto step = [&](int k, Stage& cur, Stage& nxt) {
     load_a(k - 1 - D, nxt);
     load_c(k - D, nxt);
-    const T pm1 = k > 0 ? level(k - 1, cur) : T(0);
+    T pm1 = T(0);
+    if (BUF) {
+      const T pl = level(k - 1, cur);
+      pm1 = k > 0 ? pl : T(0);
+    } else if (k > 0) {
+      pm1 = level(k - 1, cur);
+    }
     __syncthreads();
+    T acc = T(0);
     if (centre) {
       const int b0 = k % 3;
+      const T zwk = (BUF && tab) ? zcol.zw_or(k, cur.tw) : cur.zw;
+      const T zpk = (BUF && tab) ? zcol.zp_or(k, cur.tp) : cur.zp;
+      const unsigned fk = (BUF && derive) ? zcol.flag(k, nz) : cur.f;
       T c7[7];
-      inner_row<T>(cc, cur.zw, zw_below, cur.zp, cur.f, c7);
-      zw_below = cur.zw;
-      T acc = c7[0] * p0;
+      inner_row<T>(cc, zwk, zw_below, zpk, fk, c7);
+      zw_below = zwk;
+      acc = c7[0] * p0;
       acc = fmad(c7[1], lds[b0][sy][sx - 1], acc);
       acc = fmad(c7[2], lds[b0][sy][sx + 1], acc);
       acc = fmad(c7[3], lds[b0][sy - 1][sx], acc);
       acc = fmad(c7[4], lds[b0][sy + 1][sx], acc);
       acc = fmad(c7[5], pm1, acc);
       acc = fmad(c7[6], pp1, acc);
-      if (active) {
-        st_nt<T>(q, (size_t)k * P + off, acc);
-        dot = fmad(p0, acc, dot);
-      }
+      if (active) dot = fmad(p0, acc, dot);
     }
+    store(r_q, q, k, acc);
     pp1 = p0;
     p0 = pm1;
   };
@@ -1747,8 +1868,11 @@ static hipError_t fwd_h(hipStream_t st, bool fp32, bool init, const Sys* d_sys, 
 }
 
 template <typename T, int H, int D>
-static hipError_t bwd_stream(hipStream_t st, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
-  hipLaunchKernelGGL((k_bwd<T, H, D>), dim3(ntiles), dim3(bwd_threads<H>()), 0, st, d_sys, d_tiles, raw);
+static hipError_t bwd_stream(hipStream_t st, bool buf, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+  if (buf)
+    hipLaunchKernelGGL((k_bwd<T, H, D, true>), dim3(ntiles), dim3(bwd_threads<H>()), 0, st, d_sys, d_tiles, raw);
+  else
+    hipLaunchKernelGGL((k_bwd<T, H, D, false>), dim3(ntiles), dim3(bwd_threads<H>()), 0, st, d_sys, d_tiles, raw);
   return hipGetLastError();
 }
 
@@ -1764,17 +1888,22 @@ static hipError_t bwd_chunked(hipStream_t st, const Sys* d_sys, const Tile* d_ti
 //   FP64, 12-row tiles: k_bwd, 1 level ahead; 4 and 8 rows: 4 levels ahead; fewer than 256 tiles:
 //         k_bwd_ch in chunks of 7 levels (4 rows) or 3 (8 and 12 rows, the most that fit the registers
 //         without spilling) (512x4x100: 92 -> 86 us; 128x128x30: 38 -> 34 us)
+//   Each k_bwd form addresses memory through buffer descriptors when the context allows it (buf: every
+//   inner array of the launch at most kBufMaxBytes), else through 64-bit pointers (same bits). The
+//   descriptors took the 1024x768x60 mixed matvec from 282 to 271 us, the FP64 one at 12 rows from 586
+//   to 530 us, and the 512x4x100 mixed solve from 25.6 to 27.6 solves/s.
 template <int H>
-static hipError_t bwd_h(hipStream_t st, bool fp32, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+static hipError_t bwd_h(hipStream_t st, bool fp32, bool buf, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                        int raw) {
   const bool few = ntiles < kDeepTiles;
   if (fp32)
-    return few ? bwd_stream<float, H, kDeepDepth>(st, d_sys, d_tiles, ntiles, raw)
-               : bwd_stream<float, H, 1>(st, d_sys, d_tiles, ntiles, raw);
+    return few ? bwd_stream<float, H, kDeepDepth>(st, buf, d_sys, d_tiles, ntiles, raw)
+               : bwd_stream<float, H, 1>(st, buf, d_sys, d_tiles, ntiles, raw);
   if (few) return bwd_chunked<double, H, (H < 8 ? 7 : 3)>(st, d_sys, d_tiles, ntiles, raw);
   if constexpr (H < 12)
-    return bwd_stream<double, H, kDeepDepth>(st, d_sys, d_tiles, ntiles, raw);
+    return bwd_stream<double, H, kDeepDepth>(st, buf, d_sys, d_tiles, ntiles, raw);
   else
-    return bwd_stream<double, H, 1>(st, d_sys, d_tiles, ntiles, raw);
+    return bwd_stream<double, H, 1>(st, buf, d_sys, d_tiles, ntiles, raw);
 }
 
 hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
@@ -1788,12 +1917,13 @@ hipError_t launch_fwd(hipStream_t st, bool fp32, bool init, int h, const Sys* d_
   }
 }
 
-hipError_t launch_bwd(hipStream_t st, bool fp32, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles, int raw) {
+hipError_t launch_bwd(hipStream_t st, bool fp32, bool buf, int h, const Sys* d_sys, const Tile* d_tiles, int ntiles,
+                      int raw) {
   if (ntiles <= 0) return hipSuccess;
   switch (h) {
-    case 4: return bwd_h<4>(st, fp32, d_sys, d_tiles, ntiles, raw);
-    case 8: return bwd_h<8>(st, fp32, d_sys, d_tiles, ntiles, raw);
-    case 12: return bwd_h<12>(st, fp32, d_sys, d_tiles, ntiles, raw);
+    case 4: return bwd_h<4>(st, fp32, buf, d_sys, d_tiles, ntiles, raw);
+    case 8: return bwd_h<8>(st, fp32, buf, d_sys, d_tiles, ntiles, raw);
+    case 12: return bwd_h<12>(st, fp32, buf, d_sys, d_tiles, ntiles, raw);
     default: return hipErrorInvalidValue;
   }
 }

@@ -39,13 +39,13 @@ MODULE orctm_nh_c
     INTEGER(C_INT32_T) :: precision, maxit, max_outer, warm_start
     REAL(C_DOUBLE) :: rtol, rtol_inner
     INTEGER(C_INT32_T) :: check_every, device, profile, tile_rows
-    INTEGER(C_INT32_T) :: bottom_solve, zw_table, reserved(2)
+    INTEGER(C_INT32_T) :: bottom_solve, zw_table, buffer_addressing, reserved(1)
   END TYPE
 
   TYPE, BIND(C) :: orctm_nh_stats_t
     INTEGER(C_INT32_T) :: status, outer_iters, inner_iters, tile_rows
     REAL(C_DOUBLE) :: rel_res, bnorm, solve_ms, total_ms
-    INTEGER(C_INT32_T) :: zw_table, reserved_i
+    INTEGER(C_INT32_T) :: zw_table, buffer_addressing
     REAL(C_DOUBLE) :: reserved(3)
   END TYPE
 

@@ -37,12 +37,14 @@ def _dp(a: np.ndarray):
 def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optional[float] = None,
             maxit: int = 10000, max_outer: int = 40, warm_start: bool = True, check_every: int = 16,
             device: int = 0, profile: bool = False, tile_rows: int = 0, bottom_solve: bool = False,
-            zw_table: int = 0) -> abi.OptionsC:
+            zw_table: int = 0, buffer_addressing: int = 0) -> abi.OptionsC:
     """Solver options (orctm_nh_options_t). tile_rows: 0 = tuned per size class, or 4, 8, 12.
     bottom_solve: ORCTM's CG2D4NONHY option in divergence() (the 2-D bottom-pressure solve and the
     corrections of DIVG by PNH_bot). zw_table: where the iteration kernels read ZW/ZP, 0 = the
     per-level table when it reproduces every column and the launches cover at least 2^20 cells,
-    1 = the table whenever it reproduces every column, -1 = the per-cell planes."""
+    1 = the table whenever it reproduces every column, -1 = the per-cell planes. buffer_addressing:
+    0 = the matvec kernel addresses memory through buffer descriptors where every inner array fits
+    2^31 bytes, -1 = always through 64-bit pointers (same bits)."""
     lib = abi.load()
     o = abi.OptionsC()
     lib.orctm_nh_default_options(ctypes.byref(o))
@@ -58,6 +60,7 @@ def options(precision: int = abi.MIXED, rtol: float = 1e-10, rtol_inner: Optiona
     o.tile_rows = tile_rows
     o.bottom_solve = 1 if bottom_solve else 0
     o.zw_table = zw_table
+    o.buffer_addressing = buffer_addressing
     return o
 
 

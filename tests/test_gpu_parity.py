@@ -354,6 +354,55 @@ def test_chunked_and_streaming_inner_kernels_agree(precision):
     assert s4["inner_iters"] == s12["inner_iters"] and s4["outer_iters"] == s12["outer_iters"]
 
 
+# Grids for the two addressing forms of the matvec kernel: small launches (deep-prefetch form), a
+# periodic grid whose last tile column wraps (lanes past nx hold wrapped columns but must not store), the
+# transect, and a launch of 264 tiles (streaming form at 4 rows).
+ADDR_GRIDS = {
+    "regional": (SPECS["regional"], 0),
+    "transect": (SPECS["transect"], 0),
+    "wrap": (synth.GridSpec(100, 19, 8, cyclic_x=True, cyclic_y=True, smooth=3.0, seed=synth.BASE_SEED + 79), 4),
+    "streaming": (synth.GridSpec(512, 132, 6, smooth=3.0, seed=synth.BASE_SEED + 77), 4),
+}
+
+
+@pytest.mark.parametrize("name", list(ADDR_GRIDS))
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+@pytest.mark.parametrize("zw_table", [-1, 1])
+def test_buffer_and_pointer_addressing_agree(name, precision, zw_table):
+    """The matvec kernel addresses memory through buffer descriptors (masked lanes turned away by the
+    range check) or through 64-bit pointers (options.buffer_addressing = -1): the same bits for the
+    inner operator (against the oracle) and the whole solve, with the ZW/ZP table and without."""
+    print(f"[addressing] {name} precision={precision} zw_table={zw_table}", flush=True)
+    spec, rows = ADDR_GRIDS[name]
+    g, divg = synth.build(spec)
+    dt = np.float32 if precision == abi.MIXED else np.float64
+    x = synth.normal(9, 3, g.cells).astype(dt)
+    res = {}
+    for addr in (0, -1):
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=precision, tile_rows=rows, zw_table=zw_table, buffer_addressing=addr,
+                                 warm_start=False)) as s:
+            y = s.inner_apply(x)
+            st = s.solve(divg, pnh)
+        assert st["status"] == "OK", st
+        assert st["buffer_addressing"] == (1 if addr == 0 else 0), st
+        res[addr] = (y, g.interior(pnh), st)
+    (y0, p0, s0), (y1, p1, s1) = res[0], res[-1]
+    A, S = oracle.assemble(g)
+    assert bits_equal(y0, oracle.matvec(g, oracle.inner_coef(A, S, dt, grid=g), x))
+    assert bits_equal(y0, y1)
+    assert bits_equal(p0, p1)
+    assert s0["inner_iters"] == s1["inner_iters"] and s0["outer_iters"] == s1["outer_iters"]
+    assert s0["zw_table"] == s1["zw_table"]
+
+
+def test_invalid_buffer_addressing_rejected():
+    g, divg, A, S, b = case("closed")
+    with pytest.raises(Exception) as ei:
+        NHSolver(g, options(precision=abi.MIXED, buffer_addressing=1))
+    assert getattr(ei.value, "code", None) == abi.ERR_INVALID
+
+
 def test_batch_with_mixed_tile_heights_matches_alone():
     names = ["regional", "cyclic_x", "one_layer"]
     alone, solvers = {}, []
