@@ -532,16 +532,18 @@ def bench_serve(args):
         lat = [s["t_done_ms"] - s["t_submit_ms"] for s in stats]
         svc = [(s["t_done_ms"] - s["t_admit_ms"]) / nsteps for s in stats]
         by_size = {}
-        for p_, l_, v_ in zip(pick, lat, svc):
-            by_size.setdefault(p_, ([], []))
+        for p_, l_, v_, s_ in zip(pick, lat, svc, stats):
+            by_size.setdefault(p_, ([], [], []))
             by_size[p_][0].append(l_)
             by_size[p_][1].append(v_)
+            by_size[p_][2].append(s_["inner_iters"] / nsteps)
         classes = {}
         for p_ in sorted(by_size):
-            l_, v_ = by_size[p_]
+            l_, v_, it_ = by_size[p_]
             classes[f"{models[p_].grid.nx}x{models[p_].grid.ny}x{models[p_].grid.nz}"] = {
                 "requests": len(l_), "latency_ms_p50": round(pct(l_, 50), 1), "latency_ms_p99": round(pct(l_, 99), 1),
-                "service_ms_per_step_p50": round(pct(v_, 50), 2), "service_ms_per_step_p99": round(pct(v_, 99), 2)}
+                "service_ms_per_step_p50": round(pct(v_, 50), 2), "service_ms_per_step_p99": round(pct(v_, 99), 2),
+                "inner_iters_per_step": round(float(np.mean(it_)), 1)}
         cells = sum(models[p_].grid.cells for p_ in pick)
         cell_iters = sum(models[p_].grid.cells * s["inner_iters"] for p_, s in zip(pick, stats))
         return dict(n=n, rate=rate, wall=wall, lat=lat, svc=svc, classes=classes, cells=cells, cell_iters=cell_iters,
