@@ -457,7 +457,11 @@ def bench_transect(args):
         wall, lat, stats, _ = time_solves(s, args.steps, args.warmup, profiled=False)
         pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
         s.get_solution(pnh)
-        res[name] = dict(wall=wall, lat=lat, stats=stats,
+        # kernel timing (HIP events around every inner kernel, direct launches): this workload's kernel
+        # rate is what the small-launch kernel forms are tuned for
+        s.set_profile(True)
+        _, _, _, prof = time_solves(s, 2, 1)
+        res[name] = dict(wall=wall, lat=lat, stats=stats, prof=prof,
                          err=float(np.linalg.norm(g.interior(pnh) - x) / np.linalg.norm(x)),
                          rel=oracle.rel_res(g, A, g.interior(pnh), b))
         s.close()
@@ -473,6 +477,10 @@ def bench_transect(args):
         "inner_iters_mixed": mx["stats"][-1]["inner_iters"], "inner_iters_fp64": res["fp64"]["stats"][-1]["inner_iters"],
         "oracle_rel_res_mixed": mx["rel"], "oracle_rel_res_fp64": res["fp64"]["rel"],
         "zw_table": mx["stats"][-1]["zw_table"],
+        "buffer_addressing": mx["stats"][-1].get("buffer_addressing", 0),
+        "kernel_us": {k: {"matvec": round(1e3 * v["prof"]["matvec_ms"] / max(v["prof"]["matvec_launches"], 1), 2),
+                          "update": round(1e3 * v["prof"]["update_ms"] / max(v["prof"]["update_launches"], 1), 2)}
+                      for k, v in res.items()},
         "rel_err_vs_cpu_fgmres": {k: v["err"] for k, v in res.items()},
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
         "cpu_baseline": {"value": round(1.0 / cpu_s, 4), "unit": "solves/s", "cores": oracle.num_threads(),
