@@ -124,6 +124,25 @@ def host_cpu():
     return model, n
 
 
+def device_state():
+    """The GPU's clocks and power as rocm-smi reports them after the timed solves (card 0: the levels
+    its sclk/mclk/fclk lines mark as current, the power drawn and the power cap), or None where the tool
+    is absent. Rates of the same build differ from one MI355X to another; this records which one ran."""
+    import subprocess
+
+    try:
+        r = subprocess.run(["rocm-smi", "--showclocks", "--showpower", "--showmaxpower", "--showproductname",
+                            "--json"], capture_output=True, text=True, timeout=20)
+        doc = json.loads(r.stdout)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        return None
+    card = next((v for k, v in sorted(doc.items()) if k.startswith("card")), None)
+    if not isinstance(card, dict):
+        return None
+    keep = ("sclk", "mclk", "fclk", "socclk", "power", "card series", "card model", "gfx version")
+    return {k: v for k, v in card.items() if isinstance(v, str) and any(w in k.lower() for w in keep)}
+
+
 def cpu_baseline(g, divg, reps=5):
     """The oracle's FP64 PCG + line preconditioner (C, OpenMP) on the host cores: one untimed solve, then
     `reps` solves from a zero guess, each timed over its iterations only (the set-up of the inner operator,
@@ -298,6 +317,7 @@ def main():
                               "GBps": round(fwd_bytes(cells, columns, 8) / (up64_ms * 1e-3) / 1e9, 1)},
         },
         "copy_probe_GBps": round(abi.copy_probe(1 << 30, 20), 1),
+        "device": device_state(),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -421,6 +441,7 @@ def bench_batch(args):
         "max_rel_res_mixed": mx["rel"], "max_rel_res_fp64": f64["rel"] if f64 else None,
         "tile_rows_mixed": mx["heights"], "tile_rows_fp64": f64["heights"] if f64 else None,
         "latency_ms_p50": round(pct(mx["lat"], 50), 3), "latency_ms_p99": round(pct(mx["lat"], 99), 3),
+        "device": device_state(),
         "roofline": {"bound": "hbm", "kernel": "k_bwd<float> (batched)", "achieved": round(mv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch": MATVEC_BYTES[4] * cells,
