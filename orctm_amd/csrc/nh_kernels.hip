@@ -1175,8 +1175,9 @@ constexpr int kDeepDepth = 4;
 // With the table form, whether the matvec kernels derive each cell's flag bits from the column's wet
 // depths (2 B per cell less) or read the flag plane: FP64 derives (regional FP64 20.6 -> 20.9 solves/s,
 // batch 177 -> 181); FP32 reads, because the 6 more registers per thread cost its 12-row blocks their
-// second block per CU (mixed 41.3 -> 34.4, or 38.1 when held to 64 registers, with spills). The
-// chunked FP64 kernel of 12-row tiles reads too (it would spill).
+// second block per CU (mixed 41.3 -> 34.4, or 38.1 when held to 64 registers, with spills; in the
+// buffer form, held to 64 registers: 279 against 271 us per 1024x768x60 launch). The chunked FP64
+// kernel of 12-row tiles reads too (it would spill).
 template <typename T>
 constexpr bool kDeriveFlags = sizeof(T) == 8;
 
