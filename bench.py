@@ -128,11 +128,18 @@ def device_state():
     """The GPU's clocks and power as rocm-smi reports them after the timed solves (card 0: the levels
     its sclk/mclk/fclk lines mark as current, the power drawn and the power cap), or None where the tool
     is absent. Rates of the same build differ from one MI355X to another; this records which one ran."""
+    import shutil
     import subprocess
 
+    # rocm-smi is a Python script: run it with this interpreter rather than through its
+    # "#!/usr/bin/env" line, so that no process replaces itself with another (under a profiler whose
+    # library is preloaded into every child, that child has already opened the GPU)
+    tool = shutil.which("rocm-smi")
+    if tool is None:
+        return None
     try:
-        r = subprocess.run(["rocm-smi", "--showclocks", "--showpower", "--showmaxpower", "--showproductname",
-                            "--json"], capture_output=True, text=True, timeout=20)
+        r = subprocess.run([sys.executable, os.path.realpath(tool), "--showclocks", "--showpower", "--showmaxpower",
+                            "--showproductname", "--json"], capture_output=True, text=True, timeout=20)
         doc = json.loads(r.stdout)
     except (OSError, ValueError, subprocess.SubprocessError):
         return None
