@@ -452,15 +452,6 @@ void order_into(std::vector<Tile>& nat, const Sys& s, int q, int h) {
   for (const Tile& t : system_tiles(q, s.nx, s.ny, h)) nat.push_back(t);
 }
 
-// The tiles with an even bx, in the same order: one block of the paired column update runs each with
-// its right neighbour (bx + 1, when the system has one).
-std::vector<Tile> pair_tiles(const std::vector<Tile>& nat) {
-  std::vector<Tile> out;
-  for (const Tile& t : nat)
-    if (t.bx % 2 == 0) out.push_back(t);
-  return out;
-}
-
 // Scratch copies of some systems of a context, for timing the inner kernels under a tile-height
 // assignment. The kernels run a full iteration's memory traffic against scratch inner states (not
 // the first iteration, alpha = beta = 0): they overwrite the inner work vectors w, p, q and the

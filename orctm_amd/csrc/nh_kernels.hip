@@ -1010,11 +1010,11 @@ __device__ __forceinline__ T half_row_sum(T v0, T v1) {
   return v0 + v1;
 }
 
-// publish_rows for a block of two adjacent tiles (bx, bx+1) whose wavefront w holds row j0 + w of
-// both, the first tile in lanes 0-31 and the second in lanes 32-63, with their row sums already formed
-// (half_row_sum): the same slots and the same group sums per tile.
+// publish_rows for a block of two tiles whose wavefront w holds row w of both, the first tile in lanes
+// 0-31 and the second in lanes 32-63, with their row sums already formed (half_row_sum): the same
+// slots and the same group sums per tile. The second tile is (bx + 1, by), or (bx, by + 1) when vert.
 template <typename T, int H>
-__device__ __forceinline__ void publish_pair_rows(const Sys& s, int bx, int by, T a, T b) {
+__device__ __forceinline__ void publish_pair_rows(const Sys& s, int bx, int by, bool vert, T a, T b) {
   static_assert(H % RG == 0, "row groups must not straddle tiles");
   __shared__ T sh[2][2 * H];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, half = l >> 5;
@@ -1025,8 +1025,8 @@ __device__ __forceinline__ void publish_pair_rows(const Sys& s, int bx, int by, 
   }
   __syncthreads();
   if (threadIdx.x < 2 * (H / RG)) {
-    const int half2 = threadIdx.x / (H / RG), g = threadIdx.x % (H / RG), grow = by * (H / RG) + g;
-    const int tx = bx + half2;
+    const int half2 = threadIdx.x / (H / RG), g = threadIdx.x % (H / RG);
+    const int grow = (by + (vert ? half2 : 0)) * (H / RG) + g, tx = bx + (vert ? 0 : half2);
     if (grow * RG < s.ny && tx < s.tiles_x) {
       const T* v = sh[half2];
       const T sa = (v[4 * g] + v[4 * g + 1]) + (v[4 * g + 2] + v[4 * g + 3]);
@@ -1041,7 +1041,9 @@ __device__ __forceinline__ void publish_pair_rows(const Sys& s, int bx, int by, 
 
 // k_fwd over a pair of adjacent 64 x H tiles (bx even, bx+1): wavefront w holds row j0 + w of both,
 // lane l the columns 2l and 2l+1 of the pair (the first tile in lanes 0-31), so every load and store
-// is one contiguous 512-byte run per wavefront. The same operations per column, in the same order, and
+// is one contiguous 512-byte run per wavefront. A block whose tile is marked vertical (Tile::local = 1:
+// the last tiles of two row-blocks of a system with an odd number of tiles per row) holds (bx, by) in
+// lanes 0-31 and (bx, by + 1) in lanes 32-63, two runs of 256 bytes, instead of two partly filled blocks. The same operations per column, in the same order, and
 // the same row sums per tile (half_row_sum), so the same bits as k_fwd; half as many, twice as wide
 // memory operations stream the column march faster (1024x768x60: 6.0 against 5.0-5.5 TB/s for a march
 // of this shape).
@@ -1058,7 +1060,9 @@ __global__ __launch_bounds__(64 * H) void k_fwd2(const Sys* __restrict__ systems
   } else {
     if (in->done != RUN) return;
   }
-  const int i = t.bx * TX + 2 * (threadIdx.x & 63), j = t.by * H + (threadIdx.x >> 6);
+  const bool vert = t.local != 0;
+  const int l = threadIdx.x & 63;
+  const int i = t.bx * TX + 2 * (vert ? (l & 31) : l), j = (t.by + (vert ? (l >> 5) : 0)) * H + (threadIdx.x >> 6);
   const size_t P = s.plane, col = (size_t)j * s.pitch + i;
   const int nz = s.nz;
   const bool ok0 = i < s.nx && j < s.ny, ok1 = i + 1 < s.nx && j < s.ny;
@@ -1151,7 +1155,7 @@ __global__ __launch_bounds__(64 * H) void k_fwd2(const Sys* __restrict__ systems
     }
   }
   const T rho = half_row_sum(rho0, rho1), rr = half_row_sum(rr0, rr1);
-  publish_pair_rows<T, H>(s, t.bx, t.by, rho, rr);
+  publish_pair_rows<T, H>(s, t.bx, t.by, vert, rho, rr);
 }
 
 // Threads of the matvec kernel (tile of 64 x H columns): H wavefronts own the tile rows (one column

@@ -60,4 +60,37 @@ std::vector<TileT> balanced_order(const std::vector<TileT>& nat, Depth depth, in
   return out;
 }
 
+// The blocks of the paired column update, from a launch's tiles in natural order (each system's
+// tiles row-major, bx fastest). A block covers two tiles: (bx, by) with bx even and its right
+// neighbour (bx + 1, by) (local = 0). Where a system's rows hold an odd number of tiles, the last tile
+// of row-block by (even) is paired with the last tile of row-block by + 1 below it (local = 1), so two
+// partly filled blocks become one; a last tile with no partner runs alone (local = 0, no right
+// neighbour). Every tile is covered exactly once.
+template <typename TileT>
+std::vector<TileT> pair_tiles(const std::vector<TileT>& nat) {
+  std::vector<TileT> out;
+  const int n = (int)nat.size();
+  int a = 0;
+  while (a < n) {
+    int e = a, tx = 0, ty = 0;  // the system's tiles are nat[a, e)
+    while (e < n && nat[e].sys == nat[a].sys) {
+      tx = std::max(tx, nat[e].bx + 1);
+      ty = std::max(ty, nat[e].by + 1);
+      ++e;
+    }
+    for (int k = a; k < e; ++k) {
+      TileT t = nat[k];
+      if (t.bx % 2 != 0) continue;
+      t.local = 0;
+      if (t.bx + 1 == tx) {  // the last tile of an odd row
+        if (t.by % 2 != 0) continue;  // paired with the one above
+        if (t.by + 1 < ty) t.local = 1;
+      }
+      out.push_back(t);
+    }
+    a = e;
+  }
+  return out;
+}
+
 }  // namespace onh

@@ -32,6 +32,8 @@ def lib(tmp_path_factory):
     L.balanced_order.restype = ctypes.c_int
     L.xcd_order.argtypes = [p, ctypes.c_int, p]
     L.xcd_order.restype = ctypes.c_int
+    L.pair_tiles.argtypes = [p, p, ctypes.c_int, p, ctypes.c_int]
+    L.pair_tiles.restype = ctypes.c_int
     return L
 
 
@@ -109,3 +111,47 @@ def test_equal_depths_give_the_plain_xcd_runs(lib):
 def test_short_lists_keep_the_plain_order(lib):
     sys = np.repeat(np.arange(3), [40, 100, 60])
     assert np.array_equal(balanced(lib, sys, [30, 60, 45]), xcd(lib, sys))
+
+
+def pairs(lib, tx, ty):
+    tx = np.asarray(tx, dtype=np.int32)
+    ty = np.asarray(ty, dtype=np.int32)
+    cap = int((tx * ty).sum())
+    out = np.zeros(4 * cap, dtype=np.int32)
+    n = lib.pair_tiles(ptr(tx), ptr(ty), len(tx), ptr(out), cap)
+    assert n >= 0
+    return out[:4 * n].reshape(n, 4)
+
+
+@pytest.mark.parametrize("tx,ty", [(1, 1), (1, 2), (1, 7), (2, 5), (3, 1), (3, 2), (3, 9), (5, 4), (8, 3), (7, 11)])
+def test_pair_blocks_cover_every_tile_once(lib, tx, ty):
+    """Each block covers its tile and, horizontally, (bx+1, by) where that exists, or, vertically,
+    (bx, by+1); every tile of the system is covered exactly once; vertical blocks only pair the last
+    tiles of odd rows, from an even row-block down to the next."""
+    got = pairs(lib, [tx], [ty])
+    seen = np.zeros((ty, tx), dtype=int)
+    for q, bx, by, vert in got:
+        assert q == 0 and bx % 2 == 0
+        seen[by, bx] += 1
+        if vert:
+            assert tx % 2 == 1 and bx == tx - 1 and by % 2 == 0 and by + 1 < ty
+            seen[by + 1, bx] += 1
+        elif bx + 1 < tx:
+            seen[by, bx + 1] += 1
+    assert (seen == 1).all(), seen
+    half_filled = sum(1 for q, bx, by, vert in got if not vert and bx + 1 == tx)
+    assert half_filled == (ty % 2 if tx % 2 else 0)  # only the last row-block's tile of an odd row is alone
+
+
+def test_pair_blocks_of_the_64_system_batch(lib):
+    """The batch workload: per system in turn, every tile once; odd rows pair their last tiles."""
+    sizes = synth.config4_sizes()
+    tx = [-(-s.nx // 64) for s in sizes]
+    ty = [-(-s.ny // 12) for s in sizes]
+    got = pairs(lib, tx, ty)
+    assert (np.diff(got[:, 0]) >= 0).all()  # system-major, as the tiles came
+    for q in range(len(sizes)):
+        mine = got[got[:, 0] == q]
+        covered = sum(1 + (1 if v else (1 if bx + 1 < tx[q] else 0)) for _, bx, _, v in mine)
+        assert covered == tx[q] * ty[q]
+        assert len(mine) == ty[q] * (tx[q] // 2) + ((ty[q] + 1) // 2 if tx[q] % 2 else 0)

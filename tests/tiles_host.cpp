@@ -31,3 +31,21 @@ extern "C" int xcd_order(const int32_t* sys, int n, int32_t* out) {
   for (size_t b = 0; b < v.size(); ++b) out[b] = v[b].local;
   return (int)v.size();
 }
+
+// The paired column update's blocks for systems of tx[q] x ty[q] tiles (natural order): out holds
+// (system, bx, by, vertical) per block; returns the number of blocks (at most out_cap).
+extern "C" int pair_tiles(const int32_t* tx, const int32_t* ty, int nsys, int32_t* out, int out_cap) {
+  std::vector<T> nat;
+  for (int q = 0; q < nsys; ++q)
+    for (int by = 0; by < ty[q]; ++by)
+      for (int bx = 0; bx < tx[q]; ++bx) nat.push_back(T{q, bx, by, (int)nat.size()});
+  auto v = onh::pair_tiles(nat);
+  if ((int)v.size() > out_cap) return -1;
+  for (size_t b = 0; b < v.size(); ++b) {
+    out[4 * b] = v[b].sys;
+    out[4 * b + 1] = v[b].bx;
+    out[4 * b + 2] = v[b].by;
+    out[4 * b + 3] = v[b].local;
+  }
+  return (int)v.size();
+}
