@@ -435,3 +435,37 @@ def test_invalid_tile_rows_rejected():
     with pytest.raises(Exception) as ei:
         NHSolver(g, options(precision=abi.MIXED, tile_rows=6))
     assert getattr(ei.value, "code", None) == abi.ERR_INVALID
+
+
+@pytest.mark.parametrize("precision", [abi.MIXED, abi.FP64])
+def test_batch_vertical_pairs_match_alone(precision):
+    """Systems whose rows hold an odd number of tiles (300 and 140 columns: 5 and 3 tiles of 64), one of
+    them with an odd number of row-blocks, batched into one launch of 391 4-row tiles: the column update
+    pairs the last tiles of two row-blocks in one block (and runs the last one alone), where each system
+    solved on its own takes the one-tile-per-block kernel. Same bits either way."""
+    specs = [synth.GridSpec(300, 146, 12, smooth=3.0, seed=71), synth.GridSpec(140, 150, 12, smooth=3.0, seed=72),
+             synth.GridSpec(200, 90, 12, smooth=3.0, seed=73)]
+    grids = [synth.build(sp) for sp in specs]
+    alone, solvers = [], []
+    for g, divg in grids:
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        with NHSolver(g, options(precision=precision, tile_rows=4)) as s:
+            st = s.solve(divg, pnh)
+        assert st["status"] == "OK", st
+        alone.append((g.interior(pnh), st))
+        s = NHSolver(g, options(precision=precision, tile_rows=4))
+        s.set_rhs(divg)
+        solvers.append(s)
+    batch = NHBatch(solvers)
+    rc, stats = batch.solve()
+    assert rc == abi.OK, stats
+    for k, (s, st) in enumerate(zip(solvers, stats)):
+        g = s.grid
+        pnh = np.zeros((g.IE, g.JE, g.KE), order="F")
+        s.get_solution(pnh)
+        assert bits_equal(g.interior(pnh), alone[k][0]), k
+        assert st["inner_iters"] == alone[k][1]["inner_iters"]
+        assert st["rel_res"] <= RTOL
+    batch.close()
+    for s in solvers:
+        s.close()
